@@ -1,0 +1,1168 @@
+// libattract_amd: ATTRACT rigid-body docking scoring and minimization for gfx950.
+//
+// One wavefront (64 lanes) owns one pose at a time and pulls poses from a
+// device work counter until the list is exhausted (persistent kernel).  For
+// each chunk of up to 64 ligand beads:
+//   * bead phase  -- lane = bead: rigid transform in registers, grid trilinear
+//     term (one 128-byte record per bead in the corner-packed layout), voxel of
+//     the neighbour lookup, (count, start) of its receptor list;
+//   * pair phase  -- the wave walks the concatenation of all the chunk's lists
+//     in 16-byte segments of 8 receptor indices (lane = segment), so lanes stay
+//     busy whatever the per-bead list lengths; receptor beads and the pair table
+//     are read from LDS.
+// Energy and dE/dM (13 floats) are accumulated per lane and reduced with an xor
+// butterfly, which gives every lane the bit-identical sum; the ADAM update of
+// the minimizer then runs redundantly on all lanes with the pose state in
+// registers across steps.
+//
+// The file is compiled with -ffp-contract=off: every fused multiply-add below
+// is written explicitly, so the CPU twin can pin the operation order of the
+// transform and the voxelisation bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "attract_amd.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess)                                                      \
+      return fail(ATJ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kWave = 64;
+constexpr int kSeg = 8;            // receptor indices per 16-byte list segment
+constexpr int kMaxWavesPerBlock = 8;
+constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
+
+enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2 };
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value known to be identical on every lane: move it to a scalar register.
+__device__ __forceinline__ float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int t = __shfl_up(v, off, kWave);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Rotation matrix of a (not necessarily normalised) quaternion (x, y, z, w):
+// the polynomial of scipy / jax.scipy Rotation.as_matrix, used by test4.py:399.
+__device__ __forceinline__ void quat_to_mat(const float q[4], float R[9]) {
+  const float x = q[0], y = q[1], z = q[2], w = q[3];
+  const float x2 = x * x, y2 = y * y, z2 = z * z, w2 = w * w;
+  const float xy = x * y, zw = z * w, xz = x * z, yw = y * w, yz = y * z, xw = x * w;
+  R[0] = ((x2 - y2) - z2) + w2;
+  R[1] = 2.f * (xy - zw);
+  R[2] = 2.f * (xz + yw);
+  R[3] = 2.f * (xy + zw);
+  R[4] = ((-x2 + y2) - z2) + w2;
+  R[5] = 2.f * (yz - xw);
+  R[6] = 2.f * (xz - yw);
+  R[7] = 2.f * (yz + xw);
+  R[8] = ((-x2 - y2) + z2) + w2;
+}
+
+// dE/dq from A = dE/dR through the as_matrix polynomial.
+__device__ __forceinline__ void quat_grad(const float q[4], const float A[9], float g[4]) {
+  const float x = q[0], y = q[1], z = q[2], w = q[3];
+  g[0] = 2.f * (x * ((A[0] - A[4]) - A[8]) + y * (A[1] + A[3]) + z * (A[2] + A[6]) + w * (A[7] - A[5]));
+  g[1] = 2.f * (y * ((A[4] - A[0]) - A[8]) + x * (A[1] + A[3]) + z * (A[5] + A[7]) + w * (A[2] - A[6]));
+  g[2] = 2.f * (z * ((A[8] - A[0]) - A[4]) + x * (A[2] + A[6]) + y * (A[5] + A[7]) + w * (A[3] - A[1]));
+  g[3] = 2.f * (w * ((A[0] + A[4]) + A[8]) + x * (A[7] - A[5]) + y * (A[2] - A[6]) + z * (A[3] - A[1]));
+}
+
+// x'_l = ((x M0l + y M1l) + z M2l) + M3l with the first three products fused:
+// fma(z, M2l, fma(y, M1l, x*M0l)) + M3l.  The CPU twin uses the same order.
+__device__ __forceinline__ void transform(const float M[16], float x, float y, float z, float out[3]) {
+#pragma unroll
+  for (int l = 0; l < 3; ++l) out[l] = fmaf(z, M[8 + l], fmaf(y, M[4 + l], x * M[l])) + M[12 + l];
+}
+
+struct GridView {
+  float ox, oy, oz, spacing, ge, p2;
+  int dx, dy, dz, dx2, dy2, dz2, ntypes, layout;
+  const float* inner;
+  const float* outer;
+  const int2* nbgrid;
+  const uint16_t* nblist;
+};
+
+// Trilinear interpolation over one cell (test-grid-energrad.py:183-194):
+// x0 = floor(v), x1 = ceil(v), w0 = x1 - v, w1 = v - x0; corners x -> y -> z.
+// raw = sum_c w_c G_c (4 channels); dEdv = d(sum_c w_c E_c)/dv (exact mode).
+__device__ __forceinline__ void trilinear(const float* g, int layout, int t, int dx, int dy, int dz,
+                                          float vx, float vy, float vz, bool exact, float raw[4],
+                                          float dEdv[3]) {
+  const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
+  const float cx = ceilf(vx), cy = ceilf(vy), cz = ceilf(vz);
+  const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+  const float wx[2] = {cx - vx, vx - fx};
+  const float wy[2] = {cy - vy, vy - fy};
+  const float wz[2] = {cz - vz, vz - fz};
+  float4 c[8];
+  if (layout == ATJ_LAYOUT_CORNER) {
+    const size_t rec = (((size_t)t * (dx - 1) + x0) * (dy - 1) + y0) * (size_t)(dz - 1) + z0;
+    const float* p = g + rec * 32;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) c[i] = *reinterpret_cast<const float4*>(p + 4 * i);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int ix = x0 + (i >> 2), iy = y0 + ((i >> 1) & 1), iz = z0 + (i & 1);
+      const size_t idx = (((size_t)t * dx + ix) * dy + iy) * (size_t)dz + iz;
+      c[i] = *reinterpret_cast<const float4*>(g + idx * 4);
+    }
+  }
+  raw[0] = raw[1] = raw[2] = raw[3] = 0.f;
+  dEdv[0] = dEdv[1] = dEdv[2] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int ax = i >> 2, ay = (i >> 1) & 1, az = i & 1;
+    const float w = (wx[ax] * wy[ay]) * wz[az];
+    raw[0] = fmaf(w, c[i].x, raw[0]);
+    raw[1] = fmaf(w, c[i].y, raw[1]);
+    raw[2] = fmaf(w, c[i].z, raw[2]);
+    raw[3] = fmaf(w, c[i].w, raw[3]);
+    if (exact) {
+      const float sx = ax ? 1.f : -1.f, sy = ay ? 1.f : -1.f, sz = az ? 1.f : -1.f;
+      dEdv[0] = fmaf(sx * (wy[ay] * wz[az]), c[i].x, dEdv[0]);
+      dEdv[1] = fmaf(sy * (wx[ax] * wz[az]), c[i].x, dEdv[1]);
+      dEdv[2] = fmaf(sz * (wx[ax] * wy[ay]), c[i].x, dEdv[2]);
+    }
+  }
+}
+
+// Grid term of one bead at x'.  Returns E and dE/dx'; inner voxel index for the
+// neighbour lookup (-1 outside the inner grid, test-grid-energrad.py:216-218,246-253).
+__device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const float xp[3], bool do_grid,
+                                          bool exact, float& E, float g[3], int& voxel,
+                                          unsigned& in_any, float raw_out[4]) {
+  const float vx = (xp[0] - G.ox) / G.spacing;
+  const float vy = (xp[1] - G.oy) / G.spacing;
+  const float vz = (xp[2] - G.oz) / G.spacing;
+  const bool inside = (vx >= 0.f) && (vx < (float)(G.dx - 1)) && (vy >= 0.f) && (vy < (float)(G.dy - 1)) &&
+                      (vz >= 0.f) && (vz < (float)(G.dz - 1));
+  E = 0.f;
+  g[0] = g[1] = g[2] = 0.f;
+  raw_out[0] = raw_out[1] = raw_out[2] = raw_out[3] = 0.f;
+  in_any = 0;
+  voxel = -1;
+  if (inside) {
+    const int ix = (int)floorf(vx + 0.5f), iy = (int)floorf(vy + 0.5f), iz = (int)floorf(vz + 0.5f);
+    voxel = (ix * G.dy + iy) * G.dz + iz;
+  }
+  if (!do_grid || G.inner == nullptr) return;
+  float raw[4], dEdv[3];
+  float inv = 0.f;
+  if (inside) {
+    trilinear(G.inner, G.layout, gtype, G.dx, G.dy, G.dz, vx, vy, vz, exact, raw, dEdv);
+    inv = 1.f / G.spacing;
+    in_any = 1;
+  } else if (G.outer != nullptr) {
+    const float ox = (vx + G.ge) / 2.f, oy = (vy + G.ge) / 2.f, oz = (vz + G.ge) / 2.f;
+    const bool in_outer = (ox >= 0.f) && (ox < (float)(G.dx2 - 1)) && (oy >= 0.f) &&
+                          (oy < (float)(G.dy2 - 1)) && (oz >= 0.f) && (oz < (float)(G.dz2 - 1));
+    if (!in_outer) return;
+    trilinear(G.outer, G.layout, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
+    inv = 0.5f / G.spacing;
+    in_any = 1;
+  } else {
+    return;
+  }
+  E = raw[0];
+  if (exact) {
+    g[0] = dEdv[0] * inv;
+    g[1] = dEdv[1] * inv;
+    g[2] = dEdv[2] * inv;
+  } else {
+    g[0] = -raw[1];
+    g[1] = -raw[2];
+    g[2] = -raw[3];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) raw_out[i] = raw[i];
+}
+
+// ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and its dsq-derivative,
+// optionally minus the plateau energy with the force-shifted gradient (:124-147).
+// pa = (rc, ac, rmin2, ivor), pb = (emin, plateau E, 2*plateau dE/ddsq*plateaudis, -).
+// Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).
+__device__ __forceinline__ void pair_term(float dsq, const float4 pa, const float4 pb, bool plateau,
+                                          bool exact, float& E, float& gs) {
+  const float u = __builtin_amdgcn_rcpf(dsq);
+  const float u3 = (u * u) * u;
+  const float rep = pa.x * u;
+  const float vlj = (rep - pa.y) * u3;
+  const float dv = -(fmaf(4.f, rep, -3.f * pa.y) * u3) * u;
+  const bool lo = dsq < pa.z;
+  const float s = lo ? 1.f : pa.w;
+  E = lo ? fmaf(pa.w - 1.f, pb.x, vlj) : pa.w * vlj;
+  gs = 2.f * (s * dv);
+  if (plateau) {
+    E -= pb.y;
+    if (!exact) gs = fmaf(-pb.z, __builtin_amdgcn_rsqf(dsq), gs);
+  }
+}
+
+// Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs.
+__device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, float d[3]) {
+  const float u = 1.f / dsq;
+  const float u3 = (u * u) * u;
+  const bool lo = dsq < pa.z;
+  const float s = lo ? 1.f : pa.w;
+  d[0] = s * (u3 * u);
+  d[1] = -s * u3;
+  d[2] = lo ? (pa.w - 1.f) : 0.f;
+}
+
+struct WaveScratch {
+  float4 xp[kWave];   // x', y', z', ligand ff type (int bits)
+  float4 xl[kWave];   // x, y, z (local), list start (int bits)
+  int cnt[kWave];     // list length of the bead
+  int segincl[kWave]; // inclusive scan of segments per bead
+};
+
+struct KArgs {
+  GridView G;
+  const float4* rec;
+  int nrec;
+  const float* rc;
+  const float* ac;
+  const float* ivor;
+  const float* emin;
+  const float* rmin2;
+  int nrt, nlt;
+  const AtjJob* jobs;
+  const int* pose_job;
+  const int* pose_order;
+  int N;
+  // energrad / adjoint
+  const float* poses;
+  float* energy;
+  float* dEdM;
+  const float* weights;
+  float* partial;
+  // minimize
+  float* pose7;
+  float* mom1;
+  float* mom2;
+  int* count;
+  float* elast;
+  int nsteps_launch;
+  unsigned flags;
+  int* counter;
+  unsigned long long* stats;
+  int rec_in_lds;
+};
+
+// Per-pose energy and dE/dM for a uniform pose matrix M; acc[] returns the
+// wave-reduced sums on every lane.
+template <int MODE>
+__device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
+                              float* s_pgrad, WaveScratch* ws, const float4* __restrict__ lig, int nb,
+                              const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
+  float M[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
+  const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
+  const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
+  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0;
+  const bool exact = (a.flags & ATJ_GRAD_EXACT) != 0;
+  const GridView& G = a.G;
+  const float P2 = G.p2;
+#pragma unroll
+  for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
+
+  for (int c0 = 0; c0 < nb; c0 += kWave) {
+    const int j = c0 + lane;
+    int cnt = 0, start = 0;
+    float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
+    int lt = 0;
+    if (j < nb) {
+      const float4 b = lig[j];
+      xl[0] = b.x;
+      xl[1] = b.y;
+      xl[2] = b.z;
+      const int tb = __float_as_int(b.w);
+      const int gt = tb & 0xffff;
+      lt = tb >> 16;
+      transform(M, xl[0], xl[1], xl[2], xp);
+      float E, g[3], raw[4];
+      int voxel;
+      unsigned in_any;
+      bead_grid(G, gt, xp, do_grid, exact, E, g, voxel, in_any, raw);
+      acc[0] += E;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xl[k], g[l], acc[1 + 3 * k + l]);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+      st[0] += in_any;
+      if (voxel >= 0) {
+        st[1] += 1;
+        if (do_nb) {
+          const int2 e = G.nbgrid[voxel];
+          cnt = e.x;
+          start = e.y;
+          st[2] += (unsigned)cnt;
+        }
+      }
+    }
+    if (!do_nb) continue;
+    const int nseg = (cnt + kSeg - 1) / kSeg;
+    const int incl = wave_incl_scan(nseg, lane);
+    const int total = __shfl(incl, kWave - 1, kWave);
+    if (total == 0) continue;
+    ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
+    ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
+    ws->cnt[lane] = cnt;
+    ws->segincl[lane] = incl;
+    wave_sync();
+    int jb = 0;
+    for (int base = 0; base < total; base += kWave) {
+      const int item = base + lane;
+      if (item < total) {
+        while (ws->segincl[jb] <= item) ++jb;
+        const int sbase = jb ? ws->segincl[jb - 1] : 0;
+        const int seg = item - sbase;
+        const float4 bx = ws->xp[jb];
+        const float4 bl = ws->xl[jb];
+        const int bcnt = ws->cnt[jb];
+        const int bstart = __float_as_int(bl.w);
+        const int blt = __float_as_int(bx.w);
+        const int k0 = seg * kSeg;
+        const int n = min(kSeg, bcnt - k0);
+        float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+        unsigned npair = 0;
+        // two halves of 4 entries (8 bytes each) keep the number of loads in flight bounded
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+          const uint2 ent = *reinterpret_cast<const uint2*>(G.nblist + bstart + k0 + 4 * h);
+#pragma unroll
+          for (int cc = 0; cc < 4; ++cc) {
+            if (4 * h + cc < n) {
+              const unsigned wd = (cc < 2) ? ent.x : ent.y;
+              const unsigned r = (cc & 1) ? (wd >> 16) : (wd & 0xffffu);
+              const float4 rb = a.rec_in_lds ? s_rec[r] : a.rec[r];
+              const float ddx = bx.x - rb.x, ddy = bx.y - rb.y, ddz = bx.z - rb.z;
+              const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+              if (dsq < P2) {
+                const int rt = __float_as_int(rb.w);
+                const int pi = rt * a.nlt + blt;
+                const float4 pa = s_par[2 * pi];
+                const float4 pb = s_par[2 * pi + 1];
+                float E, gs;
+                pair_term(dsq, pa, pb, plateau, exact, E, gs);
+                e += E;
+                gx = fmaf(gs, ddx, gx);
+                gy = fmaf(gs, ddy, gy);
+                gz = fmaf(gs, ddz, gz);
+                ++npair;
+                if (MODE == kAdjoint) {
+                  float d[3];
+                  pair_param_grad(dsq, pa, d);
+                  if (plateau) {
+                    float dp[3];
+                    pair_param_grad(P2, pa, dp);
+                    d[0] -= dp[0];
+                    d[1] -= dp[1];
+                    d[2] -= dp[2];
+                  }
+                  const int ntp = a.nrt * a.nlt;
+                  atomicAdd(&s_pgrad[pi], wgt * d[0]);
+                  atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
+                  atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
+                }
+              }
+            }
+          }
+        }
+        st[3] += npair;
+        acc[0] += e;
+        const float g[3] = {gx, gy, gz};
+        const float xb[3] = {bl.x, bl.y, bl.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+      }
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int i = 0; i < kAcc; ++i) acc[i] = wave_sum(acc[i]);
+}
+
+__device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float M[16]) {
+  if (qt) {
+    float q[4] = {p[0], p[1], p[2], p[3]};
+    float R[9];
+    quat_to_mat(q, R);
+    M[0] = R[0]; M[1] = R[1]; M[2] = R[2]; M[3] = 1.f;
+    M[4] = R[3]; M[5] = R[4]; M[6] = R[5]; M[7] = 1.f;
+    M[8] = R[6]; M[9] = R[7]; M[10] = R[8]; M[11] = 1.f;
+    M[12] = p[4]; M[13] = p[5]; M[14] = p[6]; M[15] = 1.f;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) M[i] = p[i];
+  }
+}
+
+// Stage the receptor beads and the derived pair table into LDS.
+__device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s_pgrad) {
+  const int tid = threadIdx.x;
+  if (a.rec_in_lds)
+    for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = a.rec[i];
+  const int ntp = a.nrt * a.nlt;
+  const float P2 = a.G.p2;
+  const float P = sqrtf(P2);
+  for (int i = tid; i < ntp; i += blockDim.x) {
+    const float rc = a.rc[i], ac = a.ac[i], iv = a.ivor[i], em = a.emin[i], r2 = a.rmin2[i];
+    // plateau energy / dsq-gradient: nonbon and grad(nonbon) at plateaudis**2
+    const float u = 1.f / P2;
+    const float u3 = (u * u) * u;
+    const float rep = rc * u;
+    const float vlj = (rep - ac) * u3;
+    const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
+    const bool lo = P2 < r2;
+    const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
+    const float pG = lo ? dv : iv * dv;
+    s_par[2 * i] = make_float4(rc, ac, r2, iv);
+    s_par[2 * i + 1] = make_float4(em, pE, 2.f * pG * P, 0.f);
+  }
+  if (s_pgrad)
+    for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
+  extern __shared__ float4 smem[];
+  const int nwaves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int ntp = a.nrt * a.nlt;
+  float4* s_par = smem;
+  float4* s_rec = s_par + 2 * ntp;
+  const int nrec_lds = a.rec_in_lds ? a.nrec : 0;
+  WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
+  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
+  stage_lds(a, s_rec, s_par, s_pgrad);
+  __syncthreads();
+  WaveScratch* ws = s_ws + wave;
+  const bool qt = (a.flags & ATJ_POSE_QT) != 0;
+  unsigned st[4] = {0, 0, 0, 0};
+
+  for (;;) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(a.counter, 1);
+    k = __shfl(k, 0, kWave);
+    if (k >= a.N) break;
+    const int p = a.pose_order ? a.pose_order[k] : k;
+    const int jid = a.pose_job ? a.pose_job[p] : 0;
+    const AtjJob job = a.jobs[jid];
+    const float4* lig = reinterpret_cast<const float4*>(job.ligand.beads);
+    const int nb = job.ligand.nbeads;
+    float acc[kAcc];
+    if (MODE == kMinimize) {
+      float q[4], t[3], m[7], v[7];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = a.pose7[7 * p + i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) t[i] = a.pose7[7 * p + 4 + i];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        m[i] = a.mom1[7 * p + i];
+        v[i] = a.mom2[7 * p + i];
+      }
+      int cnt = a.count[p];
+      float elast = a.elast[p];
+      const int nst = min(a.nsteps_launch, job.nsteps - cnt);
+      for (int s = 0; s < nst; ++s) {
+        float R[9];
+        quat_to_mat(q, R);
+        const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
+                             R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
+        pose_energrad<MODE>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
+        elast = acc[0];
+        float A[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) A[i] = acc[1 + i];
+        float gr[7];
+        quat_grad(q, A, gr);
+        gr[4] = acc[10];
+        gr[5] = acc[11];
+        gr[6] = acc[12];
+        ++cnt;
+        // optax.adam: scale_by_adam (bias-corrected, eps outside the sqrt) then -lr
+        const float b1 = job.beta1, b2 = job.beta2;
+        const float bc1 = 1.f - powf(b1, (float)cnt);
+        const float bc2 = 1.f - powf(b2, (float)cnt);
+        float prm[7] = {q[0], q[1], q[2], q[3], t[0], t[1], t[2]};
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          m[i] = (1.f - b1) * gr[i] + b1 * m[i];
+          v[i] = (1.f - b2) * (gr[i] * gr[i]) + b2 * v[i];
+          const float mh = m[i] / bc1;
+          const float vh = v[i] / bc2;
+          const float upd = mh / (sqrtf(vh) + job.eps);
+          const float lr = i < 4 ? job.lr_rot : job.lr_trans;
+          prm[i] = prm[i] + (-lr) * upd;
+        }
+        // Rotation.from_quat(as_quat()): renormalise
+        const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = uni(prm[i] / nrm);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) t[i] = uni(prm[4 + i]);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          m[i] = uni(m[i]);
+          v[i] = uni(v[i]);
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a.pose7[7 * p + i] = q[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.pose7[7 * p + 4 + i] = t[i];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          a.mom1[7 * p + i] = m[i];
+          a.mom2[7 * p + i] = v[i];
+        }
+        a.count[p] = cnt;
+        a.elast[p] = elast;
+      }
+    } else {
+      float M[16];
+      load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
+      const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
+      pose_energrad<MODE>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
+      if (lane == 0) {
+        a.energy[p] = acc[0];
+        if (a.dEdM) {
+          float* o = a.dEdM + (size_t)p * 16;
+#pragma unroll
+          for (int k2 = 0; k2 < 3; ++k2) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) o[4 * k2 + l] = acc[1 + 3 * k2 + l];
+            o[4 * k2 + 3] = 0.f;
+          }
+          o[12] = acc[10];
+          o[13] = acc[11];
+          o[14] = acc[12];
+          o[15] = 0.f;
+        }
+      }
+    }
+  }
+  if (a.stats) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const unsigned s = wave_sum_u(st[i]);
+      if (lane == 0 && s) atomicAdd(a.stats + i, (unsigned long long)s);
+    }
+  }
+  if (MODE == kAdjoint) {
+    __syncthreads();
+    float* out = a.partial + (size_t)blockIdx.x * 3 * ntp;
+    for (int i = threadIdx.x; i < 3 * ntp; i += blockDim.x) out[i] = s_pgrad[i];
+  }
+}
+
+__global__ void fill_int(int* p, int v) { *p = v; }
+
+// Per-(pose, bead) terms; one thread per bead-instance (parity / debugging path).
+// With coords != NULL the items are explicit points (coords[idx], types[idx]) instead
+// of (pose, bead) pairs.
+__global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const float* coords, const int* types,
+                                  float* grid4, int* voxel_out, float* nbene, float* nbgrad) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)a.N * nb) return;
+  const bool qt = (a.flags & ATJ_POSE_QT) != 0;
+  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0;
+  const bool exact = (a.flags & ATJ_GRAD_EXACT) != 0;
+  float xp[3];
+  int tb;
+  if (coords) {
+    xp[0] = coords[3 * idx];
+    xp[1] = coords[3 * idx + 1];
+    xp[2] = coords[3 * idx + 2];
+    tb = types[idx];
+  } else {
+    const int p = (int)(idx / nb), j = (int)(idx % nb);
+    float M[16];
+    load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
+    const float4 b = lig[j];
+    tb = __float_as_int(b.w);
+    transform(M, b.x, b.y, b.z, xp);
+  }
+  const int gt = tb & 0xffff, lt = tb >> 16;
+  float E, g[3], raw[4];
+  int voxel;
+  unsigned in_any;
+  bead_grid(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
+  if (grid4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) grid4[idx * 4 + i] = raw[i];
+  }
+  if (voxel_out) voxel_out[idx] = voxel;
+  float e = 0.f, gg[3] = {0.f, 0.f, 0.f};
+  if (voxel >= 0 && (a.flags & ATJ_TERM_NB)) {
+    const int2 ent = a.G.nbgrid[voxel];
+    for (int c = 0; c < ent.x; ++c) {
+      const unsigned r = a.G.nblist[ent.y + c];
+      const float4 rb = a.rec[r];
+      const float ddx = xp[0] - rb.x, ddy = xp[1] - rb.y, ddz = xp[2] - rb.z;
+      const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+      if (dsq < a.G.p2) {
+        const int rt = __float_as_int(rb.w);
+        const int pi = rt * a.nlt + lt;
+        const float rc = a.rc[pi], ac = a.ac[pi], iv = a.ivor[pi], em = a.emin[pi], r2 = a.rmin2[pi];
+        const float P2 = a.G.p2;
+        const float u = 1.f / P2;
+        const float u3 = (u * u) * u;
+        const float rep = rc * u;
+        const float vlj = (rep - ac) * u3;
+        const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
+        const bool lo = P2 < r2;
+        const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
+        const float pG = lo ? dv : iv * dv;
+        const float4 pa = make_float4(rc, ac, r2, iv);
+        const float4 pb = make_float4(em, pE, 2.f * pG * sqrtf(P2), 0.f);
+        float Ep, gs;
+        pair_term(dsq, pa, pb, plateau, exact, Ep, gs);
+        e += Ep;
+        gg[0] = fmaf(gs, ddx, gg[0]);
+        gg[1] = fmaf(gs, ddy, gg[1]);
+        gg[2] = fmaf(gs, ddz, gg[2]);
+      }
+    }
+  }
+  if (nbene) nbene[idx] = e;
+  if (nbgrad) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) nbgrad[idx * 3 + i] = gg[i];
+  }
+}
+
+// Plain one-launch-per-step ADAM (the per-job comparison form).
+__global__ void adam_step_kernel(const float* dEdM, int N, float* pose7, float* mom1, float* mom2,
+                                 int* count, float lr_rot, float lr_trans, float b1, float b2, float eps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const float* G = dEdM + (size_t)p * 16;
+  float* P = pose7 + (size_t)p * 7;
+  float q[4] = {P[0], P[1], P[2], P[3]};
+  const float A[9] = {G[0], G[1], G[2], G[4], G[5], G[6], G[8], G[9], G[10]};
+  float gr[7];
+  quat_grad(q, A, gr);
+  gr[4] = G[12];
+  gr[5] = G[13];
+  gr[6] = G[14];
+  const int cnt = count[p] + 1;
+  const float bc1 = 1.f - powf(b1, (float)cnt);
+  const float bc2 = 1.f - powf(b2, (float)cnt);
+  float prm[7] = {q[0], q[1], q[2], q[3], P[4], P[5], P[6]};
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    float m = mom1[7 * p + i], v = mom2[7 * p + i];
+    m = (1.f - b1) * gr[i] + b1 * m;
+    v = (1.f - b2) * (gr[i] * gr[i]) + b2 * v;
+    mom1[7 * p + i] = m;
+    mom2[7 * p + i] = v;
+    const float upd = (m / bc1) / (sqrtf(v / bc2) + eps);
+    prm[i] = prm[i] + (-(i < 4 ? lr_rot : lr_trans)) * upd;
+  }
+  const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) P[i] = prm[i] / nrm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) P[4 + i] = prm[4 + i];
+  count[p] = cnt;
+}
+
+__global__ void reduce_partials_kernel(const float* partial, int blocks, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * n + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Grid construction
+// ---------------------------------------------------------------------------
+
+// Potential + force (ATTRACT sign, = -dE/dp) of ligand grid type blockIdx.y at
+// every grid point; plateau-capped inside plateaudis, zero beyond cutoff
+// (test-calc-grid-energy.py:132-148); the force of a capped receptor bead is the
+// plateau gradient projected on the plateau sphere (the complement of the
+// force-shift of test-grid-energrad.py:140-146).
+__global__ void build_potentials_kernel(GridView G, const float4* rec, int nrec, const float* rc, const float* ac,
+                                        const float* ivor, const float* emin, const float* rmin2, int nlt,
+                                        const int* lig_fftype, float cutoff2, float* out, int dx, int dy, int dz,
+                                        int outer) {
+  const int t = blockIdx.y;
+  const long long nv = (long long)dx * dy * dz;
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lt = lig_fftype[t];
+  const float P2 = G.p2;
+  const float P = sqrtf(P2);
+  if (v >= nv) return;
+  const int iz = (int)(v % dz), iy = (int)((v / dz) % dy), ix = (int)(v / ((long long)dy * dz));
+  float px, py, pz;
+  if (outer) {
+    px = G.ox + G.spacing * (2.f * ix - G.ge);
+    py = G.oy + G.spacing * (2.f * iy - G.ge);
+    pz = G.oz + G.spacing * (2.f * iz - G.ge);
+  } else {
+    px = G.ox + G.spacing * ix;
+    py = G.oy + G.spacing * iy;
+    pz = G.oz + G.spacing * iz;
+  }
+  float E = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int r = 0; r < nrec; ++r) {
+    const float4 rb = rec[r];
+    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+    if (!(dsq < cutoff2)) continue;
+    const int pi = __float_as_int(rb.w) * nlt + lt;
+    const float c_rc = rc[pi], c_ac = ac[pi], iv = ivor[pi], em = emin[pi], r2 = rmin2[pi];
+    const float dd = dsq < P2 ? P2 : dsq;
+    const float u = 1.f / dd;
+    const float u3 = (u * u) * u;
+    const float rep = c_rc * u;
+    const float vlj = (rep - c_ac) * u3;
+    const float dv = -(fmaf(4.f, rep, -3.f * c_ac) * u3) * u;
+    const bool lo = dd < r2;
+    const float e = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
+    const float dg = lo ? dv : iv * dv;
+    E += e;
+    float gs;
+    if (dsq < P2)
+      gs = dsq > 0.f ? 2.f * dg * P / sqrtf(dsq) : 0.f;
+    else
+      gs = 2.f * dg;
+    fx = fmaf(-gs, ddx, fx);
+    fy = fmaf(-gs, ddy, fy);
+    fz = fmaf(-gs, ddz, fz);
+  }
+  *reinterpret_cast<float4*>(out + ((size_t)t * nv + v) * 4) = make_float4(E, fx, fy, fz);
+}
+
+__global__ void count_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis2, int* counts) {
+  const long long nv = (long long)G.dx * G.dy * G.dz;
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int iz = (int)(v % G.dz), iy = (int)((v / G.dz) % G.dy), ix = (int)(v / ((long long)G.dy * G.dz));
+  const float px = G.ox + G.spacing * ix, py = G.oy + G.spacing * iy, pz = G.oz + G.spacing * iz;
+  int c = 0;
+  for (int r = 0; r < nrec; ++r) {
+    const float4 rb = rec[r];
+    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+    c += dsq < nbdis2;
+  }
+  counts[v] = c;
+}
+
+__global__ void fill_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis2, const int* counts,
+                                       const long long* starts, int2* nbgrid, uint16_t* nblist) {
+  const long long nv = (long long)G.dx * G.dy * G.dz;
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int iz = (int)(v % G.dz), iy = (int)((v / G.dz) % G.dy), ix = (int)(v / ((long long)G.dy * G.dz));
+  const float px = G.ox + G.spacing * ix, py = G.oy + G.spacing * iy, pz = G.oz + G.spacing * iz;
+  const long long s = starts[v];
+  const int n = counts[v];
+  int c = 0;
+  for (int r = 0; r < nrec && c < n; ++r) {
+    const float4 rb = rec[r];
+    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+    if (dsq < nbdis2) nblist[s + c++] = (uint16_t)r;
+  }
+  const int padded = (n + kSeg - 1) / kSeg * kSeg;
+  for (int k = n; k < padded; ++k) nblist[s + k] = 0xFFFFu;
+  nbgrid[v] = make_int2(n, (int)s);
+}
+
+__global__ void pack_corners_kernel(const float* plain, int ntypes, int dx, int dy, int dz, float* packed) {
+  const long long nrec = (long long)ntypes * (dx - 1) * (dy - 1) * (dz - 1);
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrec * 8) return;
+  const int c = (int)(idx & 7);
+  long long r = idx >> 3;
+  const int z = (int)(r % (dz - 1));
+  r /= (dz - 1);
+  const int y = (int)(r % (dy - 1));
+  r /= (dy - 1);
+  const int x = (int)(r % (dx - 1));
+  const int t = (int)(r / (dx - 1));
+  const int ix = x + (c >> 2), iy = y + ((c >> 1) & 1), iz = z + (c & 1);
+  const size_t src = (((size_t)t * dx + ix) * dy + iy) * (size_t)dz + iz;
+  *reinterpret_cast<float4*>(packed + idx * 4) = *reinterpret_cast<const float4*>(plain + src * 4);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
+  if (!g) return fail(ATJ_ERR_INVALID, "grid is NULL");
+  for (int i = 0; i < 3; ++i)
+    if (g->dim[i] < 2) return fail(ATJ_ERR_INVALID, "grid dim must be >= 2");
+  if (!(g->spacing > 0.f)) return fail(ATJ_ERR_INVALID, "gridspacing must be > 0");
+  if (g->layout != ATJ_LAYOUT_PLAIN && g->layout != ATJ_LAYOUT_CORNER)
+    return fail(ATJ_ERR_INVALID, "unknown grid layout");
+  if (g->outer)
+    for (int i = 0; i < 3; ++i)
+      if (g->dim2[i] < 2) return fail(ATJ_ERR_INVALID, "outer grid dim2 must be >= 2");
+  if (need_lists && (!g->nbgrid || !g->nblist)) return fail(ATJ_ERR_INVALID, "neighbour lists are NULL");
+  G.ox = g->origin[0];
+  G.oy = g->origin[1];
+  G.oz = g->origin[2];
+  G.spacing = g->spacing;
+  G.ge = g->gridextension;
+  G.p2 = g->plateaudis2;
+  G.dx = g->dim[0];
+  G.dy = g->dim[1];
+  G.dz = g->dim[2];
+  G.dx2 = g->dim2[0];
+  G.dy2 = g->dim2[1];
+  G.dz2 = g->dim2[2];
+  G.ntypes = g->ntypes;
+  G.layout = g->layout;
+  G.inner = g->inner;
+  G.outer = g->outer;
+  G.nbgrid = reinterpret_cast<const int2*>(g->nbgrid);
+  G.nblist = g->nblist;
+  return ATJ_OK;
+}
+
+int fill_common(KArgs& a, const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, uint32_t flags) {
+  if (!ff || !rec) return fail(ATJ_ERR_INVALID, "forcefield / receptor is NULL");
+  int rc = make_view(grid, a.G, (flags & ATJ_TERM_NB) != 0);
+  if (rc) return rc;
+  if ((flags & ATJ_TERM_GRID) && !grid->inner)
+    return fail(ATJ_ERR_INVALID, "grid term requested but the grid has no potentials");
+  if (ff->nrt <= 0 || ff->nlt <= 0) return fail(ATJ_ERR_INVALID, "empty pair table");
+  if (ff->nrt * ff->nlt > 4096) return fail(ATJ_ERR_UNSUPPORTED, "pair table larger than 4096 entries");
+  if (!ff->rc || !ff->ac || !ff->ivor || !ff->emin || !ff->rmin2) return fail(ATJ_ERR_INVALID, "pair table array is NULL");
+  if (rec->nrec < 0 || rec->nrec > 65535) return fail(ATJ_ERR_UNSUPPORTED, "receptor must have < 65536 beads");
+  if (rec->nrec > 0 && !rec->beads) return fail(ATJ_ERR_INVALID, "receptor beads NULL");
+  a.rec = reinterpret_cast<const float4*>(rec->beads);
+  a.nrec = rec->nrec;
+  a.rc = ff->rc;
+  a.ac = ff->ac;
+  a.ivor = ff->ivor;
+  a.emin = ff->emin;
+  a.rmin2 = ff->rmin2;
+  a.nrt = ff->nrt;
+  a.nlt = ff->nlt;
+  a.flags = flags;
+  return ATJ_OK;
+}
+
+struct DevInfo {
+  int cus = 0;
+  size_t lds_per_block = 0;
+  bool ok = false;
+};
+
+int device_info(DevInfo& d) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  static thread_local int cached_dev = -1;
+  static thread_local DevInfo cached;
+  if (cached_dev == dev && cached.ok) {
+    d = cached;
+    return ATJ_OK;
+  }
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  d.cus = p.multiProcessorCount;
+  d.lds_per_block = p.sharedMemPerBlock;
+  d.ok = true;
+  cached = d;
+  cached_dev = dev;
+  return ATJ_OK;
+}
+
+template <int MODE>
+int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
+  if (!a.counter) return fail(ATJ_ERR_INVALID, "work counter is NULL");
+  if (!a.jobs) return fail(ATJ_ERR_INVALID, "jobs is NULL");
+  DevInfo d;
+  int rc = device_info(d);
+  if (rc) return rc;
+  int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
+  if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 8");
+  const size_t ntp = (size_t)a.nrt * a.nlt;
+  const size_t base = 2 * ntp * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+  size_t lds = base + (size_t)a.nrec * sizeof(float4);
+  a.rec_in_lds = 1;
+  if (lds > d.lds_per_block) {
+    a.rec_in_lds = 0;
+    lds = base;
+  }
+  if (lds > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
+  if (lds > 65536) HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(pose_kernel<MODE>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
+  if (blocks == 0) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pose_kernel<MODE>, wpb * kWave, lds));
+    if (per_cu < 1) per_cu = 1;
+    blocks = per_cu * d.cus;
+    const long long need = ((long long)a.N + wpb - 1) / wpb;
+    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
+  }
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pose_kernel<MODE>, dim3(blocks), dim3(wpb * kWave), lds, stream, a);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* atj_last_error(void) { return g_err.c_str(); }
+
+const char* atj_version(void) { return "attract_amd 0.1 gfx950"; }
+
+int atj_energrad(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, const AtjJob* jobs_dev,
+                 int32_t njobs, const int32_t* pose_job, const int32_t* pose_order, const float* poses, int32_t N,
+                 float* energy, float* dEdM, uint32_t flags, const AtjLaunch* launch, int32_t* counter,
+                 unsigned long long* stats, void* stream) {
+  KArgs a = {};
+  int rc = fill_common(a, grid, ff, rec, flags);
+  if (rc) return rc;
+  if (N < 0 || njobs < 1) return fail(ATJ_ERR_INVALID, "N < 0 or no job");
+  if (N == 0) return ATJ_OK;
+  if (!poses || !energy) return fail(ATJ_ERR_INVALID, "poses / energy NULL");
+  a.jobs = jobs_dev;
+  a.pose_job = pose_job;
+  a.pose_order = pose_order;
+  a.N = N;
+  a.poses = poses;
+  a.energy = energy;
+  a.dEdM = dEdM;
+  a.counter = counter;
+  a.stats = stats;
+  return launch_pose_kernel<kEnergrad>(a, launch, (hipStream_t)stream);
+}
+
+int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, const AtjJob* jobs_dev,
+                 int32_t njobs, const int32_t* pose_job, const int32_t* pose_order, int32_t N,
+                 const AtjPoseState* state, int32_t nsteps_launch, uint32_t flags, const AtjLaunch* launch,
+                 int32_t* counter, unsigned long long* stats, void* stream) {
+  KArgs a = {};
+  int rc = fill_common(a, grid, ff, rec, flags & ~ATJ_POSE_QT);
+  if (rc) return rc;
+  if (N < 0 || njobs < 1 || nsteps_launch < 0) return fail(ATJ_ERR_INVALID, "N < 0, no job or nsteps < 0");
+  if (N == 0 || nsteps_launch == 0) return ATJ_OK;
+  if (!state || !state->pose || !state->m || !state->v || !state->count || !state->energy)
+    return fail(ATJ_ERR_INVALID, "pose state array NULL");
+  a.jobs = jobs_dev;
+  a.pose_job = pose_job;
+  a.pose_order = pose_order;
+  a.N = N;
+  a.pose7 = state->pose;
+  a.mom1 = state->m;
+  a.mom2 = state->v;
+  a.count = state->count;
+  a.elast = state->energy;
+  a.nsteps_launch = nsteps_launch;
+  a.counter = counter;
+  a.stats = stats;
+  return launch_pose_kernel<kMinimize>(a, launch, (hipStream_t)stream);
+}
+
+int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                         const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job, const float* poses,
+                         int32_t N, const float* weights, float* energy, float* dEdM, float* partial,
+                         uint32_t flags, const AtjLaunch* launch, int32_t* counter, void* stream) {
+  KArgs a = {};
+  int rc = fill_common(a, grid, ff, rec, flags);
+  if (rc) return rc;
+  if (N < 0 || njobs < 1) return fail(ATJ_ERR_INVALID, "N < 0 or no job");
+  if (!launch || launch->blocks <= 0) return fail(ATJ_ERR_INVALID, "adjoint needs launch->blocks > 0");
+  if (!poses || !energy || !weights || !partial) return fail(ATJ_ERR_INVALID, "NULL argument");
+  a.jobs = jobs_dev;
+  a.pose_job = pose_job;
+  a.N = N;
+  a.poses = poses;
+  a.energy = energy;
+  a.dEdM = dEdM;
+  a.weights = weights;
+  a.partial = partial;
+  a.counter = counter;
+  return launch_pose_kernel<kAdjoint>(a, launch, (hipStream_t)stream);
+}
+
+int atj_reduce_partials(const float* partial, int32_t blocks, int32_t n, float* out, void* stream) {
+  if (!partial || !out || blocks <= 0 || n <= 0) return fail(ATJ_ERR_INVALID, "bad reduce arguments");
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+                     blocks, n, out);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_bead_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, const AtjLigand* ligand,
+                   const float* poses, int32_t N, float* grid4, int32_t* voxel, float* nbene, float* nbgrad,
+                   uint32_t flags, void* stream) {
+  KArgs a = {};
+  int rc = fill_common(a, grid, ff, rec, flags);
+  if (rc) return rc;
+  if (!ligand || !poses) return fail(ATJ_ERR_INVALID, "ligand / poses NULL");
+  a.N = N;
+  a.poses = poses;
+  const long long tot = (long long)N * ligand->nbeads;
+  if (tot == 0) return ATJ_OK;
+  hipLaunchKernelGGL(bead_terms_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     reinterpret_cast<const float4*>(ligand->beads), ligand->nbeads, nullptr, nullptr, grid4, voxel,
+                     nbene, nbgrad);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_point_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, const float* coords,
+                    const int32_t* types, int32_t M, float* grid4, int32_t* voxel, float* nbene, float* nbgrad,
+                    uint32_t flags, void* stream) {
+  KArgs a = {};
+  int rc = fill_common(a, grid, ff, rec, flags);
+  if (rc) return rc;
+  if (M < 0) return fail(ATJ_ERR_INVALID, "M < 0");
+  if (M == 0) return ATJ_OK;
+  if (!coords || !types) return fail(ATJ_ERR_INVALID, "coords / types NULL");
+  a.N = M;
+  hipLaunchKernelGGL(bead_terms_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     nullptr, 1, coords, types, grid4, voxel, nbene, nbgrad);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_adam_step(const float* dEdM, int32_t N, const AtjPoseState* state, float lr_rot, float lr_trans,
+                  float beta1, float beta2, float eps, void* stream) {
+  if (N <= 0) return N == 0 ? ATJ_OK : fail(ATJ_ERR_INVALID, "N < 0");
+  if (!dEdM || !state || !state->pose || !state->m || !state->v || !state->count)
+    return fail(ATJ_ERR_INVALID, "NULL argument");
+  hipLaunchKernelGGL(adam_step_kernel, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, dEdM, N,
+                     state->pose, state->m, state->v, state->count, lr_rot, lr_trans, beta1, beta2,
+                     eps);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff, const AtjReceptor* rec,
+                              const int32_t* lig_fftype_dev, float cutoff, float* inner_out, float* outer_out,
+                              void* stream) {
+  GridView G;
+  int rc = make_view(geom, G, false);
+  if (rc) return rc;
+  if (!ff || !rec || !lig_fftype_dev || !inner_out) return fail(ATJ_ERR_INVALID, "NULL argument");
+  if (geom->ntypes <= 0 || geom->ntypes > 65535) return fail(ATJ_ERR_INVALID, "bad ntypes");
+  const float c2 = cutoff * cutoff;
+  const float4* rb = reinterpret_cast<const float4*>(rec->beads);
+  long long nv = (long long)G.dx * G.dy * G.dz;
+  hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
+                     (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2, ff->nlt,
+                     lig_fftype_dev, c2, inner_out, G.dx, G.dy, G.dz, 0);
+  HIPCHK(hipGetLastError());
+  if (outer_out) {
+    nv = (long long)G.dx2 * G.dy2 * G.dz2;
+    hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
+                       (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2,
+                       ff->nlt, lig_fftype_dev, c2, outer_out, G.dx2, G.dy2, G.dz2, 1);
+    HIPCHK(hipGetLastError());
+  }
+  return ATJ_OK;
+}
+
+int atj_grid_count_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis, int32_t* counts,
+                              void* stream) {
+  GridView G;
+  int rc = make_view(geom, G, false);
+  if (rc) return rc;
+  if (!rec || !counts) return fail(ATJ_ERR_INVALID, "NULL argument");
+  const long long nv = (long long)G.dx * G.dy * G.dz;
+  hipLaunchKernelGGL(count_neighbours_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis * neighbourdis, counts);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_grid_fill_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis, const int32_t* counts,
+                             const int64_t* starts, int32_t* nbgrid, uint16_t* nblist, void* stream) {
+  GridView G;
+  int rc = make_view(geom, G, false);
+  if (rc) return rc;
+  if (!rec || !counts || !starts || !nbgrid || !nblist) return fail(ATJ_ERR_INVALID, "NULL argument");
+  const long long nv = (long long)G.dx * G.dy * G.dz;
+  hipLaunchKernelGGL(fill_neighbours_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis * neighbourdis, counts,
+                     reinterpret_cast<const long long*>(starts), reinterpret_cast<int2*>(nbgrid), nblist);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_grid_pack_corners(const float* plain, int32_t ntypes, const int32_t dim[3], float* packed, void* stream) {
+  if (!plain || !packed || !dim || ntypes <= 0) return fail(ATJ_ERR_INVALID, "NULL argument");
+  for (int i = 0; i < 3; ++i)
+    if (dim[i] < 2) return fail(ATJ_ERR_INVALID, "dim must be >= 2");
+  const long long n = (long long)ntypes * (dim[0] - 1) * (dim[1] - 1) * (dim[2] - 1) * 8;
+  hipLaunchKernelGGL(pack_corners_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plain,
+                     ntypes, dim[0], dim[1], dim[2], packed);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+}  // extern "C"

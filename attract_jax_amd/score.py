@@ -1,0 +1,250 @@
+"""Reference-named scoring entry points, computed by the HIP kernels.
+
+Names, argument order and meaning follow the reference scripts:
+  potential_energrad / potential_energy   test-grid-energrad.py:214-244
+  neighbour_energy                         test-grid-energrad.py:262-296
+  main / value_and_grad_main               test-grid-energrad.py:298-317
+  main_quat / value_and_grad_quat          test4.py:389-401,415
+`grid` may be a reference Grid namedtuple (read_grid / read_grid2 / pad_grid output,
+neighbour indices remapped) or a DeviceGrid; `ff` a namedtuple with fields rc, ac,
+ivor, emin, rmin2 (compacted, test-grid-energrad.py:67-79) or a ForceField.  Results
+are float32 torch tensors on the GPU.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import DeviceGrid, ForceField, Ligand, Receptor, _ptr, _stream
+
+_grid_cache = {}
+
+
+def device_grid(grid, layout=_lib.ATJ_LAYOUT_CORNER):
+    """DeviceGrid of a reference Grid, uploaded once per grid object."""
+    if isinstance(grid, DeviceGrid):
+        return grid
+    key = (id(grid), layout)
+    hit = _grid_cache.get(key)
+    if hit is not None and hit[0] is grid:
+        return hit[1]
+    if len(_grid_cache) > 8:
+        _grid_cache.clear()
+    dg = DeviceGrid.from_grid(grid, layout=layout)
+    _grid_cache[key] = (grid, dg)
+    return dg
+
+
+def device_ff(ff):
+    if isinstance(ff, ForceField):
+        return ff
+    return ForceField.create(ff.rc, ff.ac, ff.ivor, ff.emin, ff.rmin2)
+
+
+def _f32(x, device):
+    if isinstance(x, torch.Tensor):
+        return x.detach().to(device=device, dtype=torch.float32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(x), dtype=np.float32), device=device)
+
+
+def _default_flags(dg, terms):
+    flags = 0
+    if "grid" in terms and dg.has_potentials:
+        flags |= _lib.ATJ_TERM_GRID
+    if "nb" in terms:
+        flags |= _lib.ATJ_TERM_NB
+        if dg.has_potentials:
+            flags |= _lib.ATJ_NB_PLATEAU
+    return flags
+
+
+def point_terms(coords, grid_types, ff_types, grid, ff=None, receptor=None, flags=None, terms=("grid", "nb")):
+    """Per-point grid4 (M,4), voxel (M,), nb energy (M,), nb gradient (M,3) at explicit coordinates."""
+    lib = _lib.load()
+    dg = device_grid(grid)
+    d = dg.device
+    c = _f32(coords, d).reshape(-1, 3)
+    M = c.shape[0]
+    gt = np.asarray(grid_types, np.int64).reshape(-1)
+    lt = np.asarray(ff_types, np.int64).reshape(-1) if ff_types is not None else np.zeros_like(gt)
+    types = torch.as_tensor((gt | (lt << 16)).astype(np.int32), device=d)
+    if flags is None:
+        flags = _default_flags(dg, terms)
+    if ff is None:
+        ff = ForceField.create(np.ones((1, 1)), np.ones((1, 1)), np.ones((1, 1)))
+    ffd = device_ff(ff)
+    if receptor is None:
+        receptor = Receptor(torch.zeros((1, 4), device=d), 0)
+    grid4 = torch.zeros((M, 4), dtype=torch.float32, device=d)
+    voxel = torch.empty(M, dtype=torch.int32, device=d)
+    nbe = torch.zeros(M, dtype=torch.float32, device=d)
+    nbg = torch.zeros((M, 3), dtype=torch.float32, device=d)
+    _lib.check(lib.atj_point_terms(ctypes.byref(dg.struct()), ctypes.byref(ffd.struct()),
+                                   ctypes.byref(receptor.struct()), _ptr(c), _ptr(types), M, _ptr(grid4),
+                                   _ptr(voxel), _ptr(nbe), _ptr(nbg), flags, _stream(d)))
+    return grid4, voxel, nbe, nbg
+
+
+def potential_energrad(all_coors_lig, lig_atomtype_pos, grid):
+    """(N, nlig, 4) interpolated [E, gx, gy, gz] of each bead (stored ATTRACT sign)."""
+    c = _f32(all_coors_lig, device_grid(grid).device)
+    N, nlig = c.shape[0], c.shape[1]
+    gt = np.broadcast_to(np.asarray(lig_atomtype_pos).reshape(1, nlig), (N, nlig))
+    grid4, _, _, _ = point_terms(c, gt, None, grid, flags=_lib.ATJ_TERM_GRID)
+    return grid4.reshape(N, nlig, 4)
+
+
+def potential_energy(all_coors_lig, lig_atomtype_pos, grid):
+    """(N,) sum over beads of the interpolated grid energy."""
+    return potential_energrad(all_coors_lig, lig_atomtype_pos, grid)[:, :, 0].sum(dim=1)
+
+
+def neighbour_energy(coor_rec, rec_atomtypes, all_coors_lig, lig_atomtypes, ff, grid, plateau=None):
+    """(N,) neighbour-list pair energy.  plateau defaults to 'grid has potentials'
+    (test-grid-energrad.py:129-131 vs test-nb-energrad.py:142-145)."""
+    dg = device_grid(grid)
+    c = _f32(all_coors_lig, dg.device)
+    N, nlig = c.shape[0], c.shape[1]
+    if plateau is None:
+        plateau = dg.has_potentials
+    flags = _lib.ATJ_TERM_NB | (_lib.ATJ_NB_PLATEAU if plateau else 0)
+    rec = Receptor.create(coor_rec, rec_atomtypes, dg.device)
+    lt = np.broadcast_to(np.asarray(lig_atomtypes).reshape(1, nlig), (N, nlig))
+    _, _, nbe, _ = point_terms(c, np.zeros_like(lt), lt, dg, ff, rec, flags=flags)
+    return nbe.reshape(N, nlig).sum(dim=1)
+
+
+class ScoringSystem:
+    """Receptor grid, pair table and receptor beads resident on the device, plus a
+    work-counter workspace: the state the energrad / minimize launches share."""
+
+    def __init__(self, grid, ff, receptor, flags=None, launch=None):
+        self.grid = device_grid(grid)
+        self.ff = device_ff(ff)
+        self.receptor = receptor
+        self.device = self.grid.device
+        self.flags = flags if flags is not None else _default_flags(self.grid, ("grid", "nb"))
+        self.launch = launch or _lib.AtjLaunch(8, 0)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.stats = torch.zeros(_lib.ATJ_NSTATS, dtype=torch.int64, device=self.device)
+        self._gs = self.grid.struct()
+        self._fs = self.ff.struct()
+        self._rs = receptor.struct()
+
+    def refresh(self):
+        self._gs = self.grid.struct()
+        self._fs = self.ff.struct()
+        self._rs = self.receptor.struct()
+
+    def jobs_tensor(self, ligands, nsteps=0, lr_rot=1e-3, lr_trans=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+                    pose_counts=None):
+        """Device array of AtjJob records for the given ligands."""
+        if not isinstance(ligands, (list, tuple)):
+            ligands = [ligands]
+        arr = (_lib.AtjJob * len(ligands))()
+        begin = 0
+        for i, lig in enumerate(ligands):
+            n = int(pose_counts[i]) if pose_counts is not None else 0
+            ns = int(nsteps[i]) if np.ndim(nsteps) else int(nsteps)
+            lrr = float(lr_rot[i]) if np.ndim(lr_rot) else float(lr_rot)
+            lrt = float(lr_trans[i]) if np.ndim(lr_trans) else float(lr_trans)
+            arr[i] = _lib.AtjJob(lig.struct(), begin, n, ns, lrr, lrt, beta1, beta2, eps)
+            begin += n
+        raw = np.frombuffer(bytes(arr), np.uint8)
+        return torch.from_numpy(raw.copy()).to(self.device)
+
+    def energrad(self, poses, ligand, *, pose_job=None, jobs=None, njobs=1, order=None, grad=True, flags=None,
+                 stats=False, qt=False, stream=None):
+        """energies (N,), dE/dM (N,4,4) of poses (N,4,4) [or (N,7) with qt=True]."""
+        lib = _lib.load()
+        d = self.device
+        P = _f32(poses, d)
+        N = P.shape[0]
+        P = P.reshape(N, 7 if qt else 16)
+        if jobs is None:
+            jobs = self.jobs_tensor([ligand])
+            njobs = 1
+        energy = torch.empty(N, dtype=torch.float32, device=d)
+        dEdM = torch.empty((N, 4, 4), dtype=torch.float32, device=d) if grad else None
+        fl = (self.flags if flags is None else flags) | (_lib.ATJ_POSE_QT if qt else 0)
+        st = stream if stream is not None else _stream(d)
+        _lib.check(lib.atj_energrad(ctypes.byref(self._gs), ctypes.byref(self._fs), ctypes.byref(self._rs),
+                                    _ptr(jobs), njobs, _ptr(pose_job), _ptr(order), _ptr(P), N, _ptr(energy),
+                                    _ptr(dEdM), fl, ctypes.byref(self.launch), _ptr(self.counter),
+                                    _ptr(self.stats) if stats else ctypes.c_void_p(0), st))
+        return energy, dEdM
+
+    def bead_terms(self, poses, ligand, flags=None, qt=False):
+        lib = _lib.load()
+        d = self.device
+        P = _f32(poses, d)
+        N = P.shape[0]
+        P = P.reshape(N, 7 if qt else 16)
+        nb = ligand.nbeads
+        grid4 = torch.zeros((N, nb, 4), dtype=torch.float32, device=d)
+        voxel = torch.empty((N, nb), dtype=torch.int32, device=d)
+        nbe = torch.zeros((N, nb), dtype=torch.float32, device=d)
+        nbg = torch.zeros((N, nb, 3), dtype=torch.float32, device=d)
+        fl = (self.flags if flags is None else flags) | (_lib.ATJ_POSE_QT if qt else 0)
+        _lib.check(lib.atj_bead_terms(ctypes.byref(self._gs), ctypes.byref(self._fs), ctypes.byref(self._rs),
+                                      ctypes.byref(ligand.struct()), _ptr(P), N, _ptr(grid4), _ptr(voxel),
+                                      _ptr(nbe), _ptr(nbg), fl, _stream(d)))
+        return grid4, voxel, nbe, nbg
+
+
+def _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    dg = device_grid(grid)
+    rec = Receptor.create(coor_rec, rec_atomtypes, dg.device)
+    lig = Ligand.create(coor_lig, lig_atomtype_pos, lig_atomtypes, dg.device)
+    return ScoringSystem(dg, device_ff(ff), rec), lig
+
+
+def main(mat, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    """(sum of energies, energies (N,)) of the poses mat (N,4,4) (test-grid-energrad.py:298-306)."""
+    sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
+    e, _ = sysm.energrad(mat, lig, grad=False)
+    return e.sum(), e
+
+
+def value_and_grad_main(mat, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    """((total, energies), dE/dmat (N,4,4)) -- jax.value_and_grad(main, has_aux=True)
+    with the ATTRACT gradient convention (test-grid-energrad.py:316-327)."""
+    sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
+    e, g = sysm.energrad(mat, lig, grad=True)
+    return (e.sum(), e), g
+
+
+def quat_grad_from_dEdM(quat, dEdM):
+    """dE/dquat (N,4) through Rotation.as_matrix from dE/dM (torch, float32)."""
+    q = quat
+    A = dEdM[:, :3, :3]
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    g0 = 2 * (x * (A[:, 0, 0] - A[:, 1, 1] - A[:, 2, 2]) + y * (A[:, 0, 1] + A[:, 1, 0])
+              + z * (A[:, 0, 2] + A[:, 2, 0]) + w * (A[:, 2, 1] - A[:, 1, 2]))
+    g1 = 2 * (y * (A[:, 1, 1] - A[:, 0, 0] - A[:, 2, 2]) + x * (A[:, 0, 1] + A[:, 1, 0])
+              + z * (A[:, 1, 2] + A[:, 2, 1]) + w * (A[:, 0, 2] - A[:, 2, 0]))
+    g2 = 2 * (z * (A[:, 2, 2] - A[:, 0, 0] - A[:, 1, 1]) + x * (A[:, 0, 2] + A[:, 2, 0])
+              + y * (A[:, 1, 2] + A[:, 2, 1]) + w * (A[:, 1, 0] - A[:, 0, 1]))
+    g3 = 2 * (w * (A[:, 0, 0] + A[:, 1, 1] + A[:, 2, 2]) + x * (A[:, 2, 1] - A[:, 1, 2])
+              + y * (A[:, 0, 2] - A[:, 2, 0]) + z * (A[:, 1, 0] - A[:, 0, 1]))
+    return torch.stack([g0, g1, g2, g3], dim=1)
+
+
+def main_quat(quat, trans, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    """test4.py:396-401 with a single (non-conformer) ligand: (total, energies)."""
+    sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
+    d = sysm.device
+    qt = torch.cat([_f32(quat, d), _f32(trans, d)], dim=1)
+    e, _ = sysm.energrad(qt, lig, grad=False, qt=True)
+    return e.sum(), e
+
+
+def value_and_grad_quat(quat, trans, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    """((total, energies), (dE/dquat (N,4), dE/dtrans (N,3))) -- test4.py:415."""
+    sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
+    d = sysm.device
+    q = _f32(quat, d)
+    qt = torch.cat([q, _f32(trans, d)], dim=1)
+    e, g = sysm.energrad(qt, lig, grad=True, qt=True)
+    return (e.sum(), e), (quat_grad_from_dEdM(q, g), g[:, 3, :3].contiguous())

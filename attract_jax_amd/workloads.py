@@ -1,0 +1,212 @@
+"""Synthetic docking systems of the five benchmark configurations.
+
+All randomness is numpy.random.default_rng(1000 * config + stream).  Receptor
+beads: random sequential addition inside a sphere with a minimum separation.
+Pair table: symmetric 32x32 with rmin ~ U(3, 5) A, |emin| ~ U(0.1, 1.5), a pair
+repulsive (ivor = -1) with probability 0.1; mapped to the ATTRACT parameters
+rbc = rmin sqrt(3)/2, abc = 256/27 |emin|, rc = abc rbc^8, ac = abc rbc^6
+(attract-original/convert-attract-par.py:16-22).  Bead types are 1..32; the pair
+table and the potential grids are indexed by type - 1.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import pose as posemod
+
+NTYPES = 32
+
+
+def rng_for(config, stream):
+    return np.random.default_rng(1000 * config + stream)
+
+
+def random_receptor(n, radius, min_sep, rng, max_tries=200000):
+    """n points uniformly inside a sphere, pairwise >= min_sep apart (sequential addition)."""
+    pts = np.empty((n, 3))
+    k = 0
+    tries = 0
+    ms2 = min_sep * min_sep
+    while k < n:
+        batch = rng.uniform(-radius, radius, size=(256, 3))
+        batch = batch[(batch * batch).sum(1) <= radius * radius]
+        for c in batch:
+            tries += 1
+            if k == 0 or ((pts[:k] - c) ** 2).sum(1).min() >= ms2:
+                pts[k] = c
+                k += 1
+                if k == n:
+                    break
+        if tries > max_tries:
+            raise RuntimeError("random sequential addition did not converge; lower min_sep")
+    return pts
+
+
+def random_pair_table(rng, ntypes=NTYPES, p_repulsive=0.1):
+    """(rc, ac, ivor, rmin, emin_abs) symmetric ntypes x ntypes float64 tables."""
+    iu = np.triu_indices(ntypes)
+    rmin = np.zeros((ntypes, ntypes))
+    eabs = np.zeros((ntypes, ntypes))
+    ivor = np.ones((ntypes, ntypes))
+    rmin[iu] = rng.uniform(3.0, 5.0, size=len(iu[0]))
+    eabs[iu] = rng.uniform(0.1, 1.5, size=len(iu[0]))
+    ivor[iu] = np.where(rng.random(len(iu[0])) < p_repulsive, -1.0, 1.0)
+    for a in (rmin, eabs, ivor):
+        a.T[iu] = a[iu]
+    rc, ac = rc_ac_from_rmin_emin(rmin, eabs)
+    return rc, ac, ivor, rmin, eabs
+
+
+def rc_ac_from_rmin_emin(rmin, eabs):
+    rbc = rmin * np.sqrt(3.0) / 2.0
+    abc = 256.0 / 27.0 * np.abs(eabs)
+    return abc * rbc ** 8, abc * rbc ** 6
+
+
+def random_chain(n, rng, bond=3.8, min_sep=3.0):
+    """A compact self-avoiding bead chain of n beads, centred on its centroid."""
+    pts = [np.zeros(3)]
+    while len(pts) < n:
+        for _ in range(1000):
+            d = rng.normal(size=3)
+            d *= bond / np.linalg.norm(d)
+            c = pts[-1] + d
+            others = np.array(pts[:-1])
+            if len(others) == 0 or ((others - c) ** 2).sum(1).min() >= min_sep ** 2:
+                # keep the chain compact: prefer steps towards the centroid
+                if len(pts) > 3 and np.linalg.norm(c - np.mean(pts, 0)) > 1.3 * bond * len(pts) ** (1 / 3):
+                    continue
+                pts.append(c)
+                break
+        else:
+            raise RuntimeError("chain growth failed")
+    pts = np.array(pts)
+    return pts - pts.mean(0)
+
+
+LEUCINE_LIKE = np.array([[-1.45, 0.55, 0.30],   # N
+                         [0.00, 0.70, 0.10],    # CA
+                         [0.55, 2.10, 0.35],    # C
+                         [0.80, -0.45, -0.55]])  # side-chain bead (CSE-like)
+
+
+@dataclass
+class LigandSpec:
+    coor: np.ndarray     # (nbeads, 3) local coordinates
+    types: np.ndarray    # (nbeads,) 1..32
+
+
+@dataclass
+class Workload:
+    name: str
+    config: int
+    coor_rec: np.ndarray
+    rec_types: np.ndarray
+    rc: np.ndarray
+    ac: np.ndarray
+    ivor: np.ndarray
+    rmin: np.ndarray
+    emin_abs: np.ndarray
+    grid_kw: dict
+    ligands: list
+    pose_counts: list
+    quat: np.ndarray
+    trans: np.ndarray
+    nsteps: int = 0
+    lr_rot: float = 1e-3
+    lr_trans: float = 1e-3
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def npose(self):
+        return int(sum(self.pose_counts))
+
+
+def c1(npose=1000):
+    """Plain method, small: 300 beads in R=15 A (min sep 3 A); inner grid 32^3 @0.9 A
+    with an outer 32^3 @1.8 A (gridextension 16) so that ligand centres at 18-25 A
+    lie inside a grid; neighbour lists within 5 A (plateau 4.2 A = 5 A minus half a
+    voxel diagonal); 4-bead leucine-like ligand; 100 ADAM steps, lr 0.1 / 0.01."""
+    rng = rng_for(1, 0)
+    coor_rec = random_receptor(300, 15.0, 3.0, rng)
+    rec_types = rng_for(1, 1).integers(1, NTYPES + 1, size=300)
+    rc, ac, ivor, rmin, eabs = random_pair_table(rng_for(1, 2))
+    lig = LigandSpec(LEUCINE_LIKE.copy(), rng_for(1, 3).integers(1, NTYPES + 1, size=4))
+    q, t = posemod.random_poses(npose, rng_for(1, 4), 18.0, 25.0)
+    grid_kw = dict(origin=(-14.4, -14.4, -14.4), spacing=0.9, dim=(32, 32, 32), dim2=(32, 32, 32),
+                   gridextension=16, plateaudis=4.2, neighbourdis=5.0)
+    return Workload("C1 plain small: 300-bead receptor, 32^3 grid, 4-bead ligand", 1, coor_rec, rec_types, rc, ac,
+                    ivor, rmin, eabs, grid_kw, [lig], [npose], q, t, nsteps=100, lr_rot=0.01, lr_trans=0.1)
+
+
+def _c2_receptor():
+    rng = rng_for(2, 0)
+    coor_rec = random_receptor(2000, 25.0, 2.25, rng)
+    rec_types = rng_for(2, 1).integers(1, NTYPES + 1, size=2000)
+    rc, ac, ivor, rmin, eabs = random_pair_table(rng_for(2, 2))
+    grid_kw = dict(origin=(-43.2, -43.2, -43.2), spacing=0.9, dim=(96, 96, 96), dim2=(64, 64, 64),
+                   gridextension=16, plateaudis=10.0, neighbourdis=10.8)
+    return coor_rec, rec_types, rc, ac, ivor, rmin, eabs, grid_kw
+
+
+def c2(npose=131072):
+    """Single-shot energrad at scale: 2000 beads in R=25 A, inner 96^3 @0.9 A + outer
+    64^3 @1.8 A, plateau 10 A / lists 10.8 A; 21-bead trinucleotide-like ligand;
+    centres at 25-35 A."""
+    coor_rec, rec_types, rc, ac, ivor, rmin, eabs, grid_kw = _c2_receptor()
+    lig = LigandSpec(random_chain(21, rng_for(2, 3)), rng_for(2, 4).integers(1, NTYPES + 1, size=21))
+    q, t = posemod.random_poses(npose, rng_for(2, 5), 25.0, 35.0)
+    return Workload("C2 energrad: 2000-bead receptor, 96^3+64^3 grids, 21-bead ligand", 2, coor_rec, rec_types, rc,
+                    ac, ivor, rmin, eabs, grid_kw, [lig], [npose], q, t)
+
+
+def c3(nligands=64, poses_per_ligand=4096, nsteps=200):
+    """Batching: the C2 receptor grid, 64 ligands of U{5..60} beads, 4096 poses each,
+    200 fused ADAM steps (lr 1e-3, the reference's single rate)."""
+    coor_rec, rec_types, rc, ac, ivor, rmin, eabs, grid_kw = _c2_receptor()
+    r = rng_for(3, 0)
+    sizes = r.integers(5, 61, size=nligands)
+    ligs = [LigandSpec(random_chain(int(n), rng_for(3, 100 + i)), rng_for(3, 200 + i).integers(1, NTYPES + 1, size=n))
+            for i, n in enumerate(sizes)]
+    q, t = posemod.random_poses(nligands * poses_per_ligand, rng_for(3, 1), 25.0, 35.0)
+    return Workload("C3 batched minimization: 64 ligands x 4096 poses, 5-60 beads, 200 ADAM steps", 3, coor_rec,
+                    rec_types, rc, ac, ivor, rmin, eabs, grid_kw, ligs, [poses_per_ligand] * nligands, q, t,
+                    nsteps=nsteps, lr_rot=1e-3, lr_trans=1e-3)
+
+
+def c4(npose=65536):
+    """Differentiation: C2 energy, 65536 poses, d/d(pose DOFs) and d/d(rmin, emin)."""
+    w = c2(npose)
+    w.name = "C4 adjoint: C2 system, 65536 poses"
+    w.config = 4
+    return w
+
+
+def serving_jobs(njobs, seed=5000, rate=20.0):
+    """Config-5 job stream: Poisson arrivals (mean `rate` jobs/s), receptors of
+    500-4000 beads, ligands of 5-60 beads, log-uniform 1e3..2e5 poses, 100-300 steps."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        nrec = int(rng.integers(500, 4001))
+        jobs.append(dict(arrival=t, nrec=nrec, nlig=int(rng.integers(5, 61)),
+                         nposes=int(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))),
+                         nsteps=int(rng.integers(100, 301)), seed=seed + 1 + i))
+    return jobs
+
+
+def build_system(w, device=None, layout=None, keep_plain=False, flags=None, launch=None):
+    """Device objects of a workload: (ScoringSystem, [Ligand]); the grid is computed on the GPU."""
+    from . import _lib
+    from .device import DeviceGrid, ForceField, Ligand, Receptor
+    from .score import ScoringSystem
+    if layout is None:
+        layout = _lib.ATJ_LAYOUT_CORNER
+    ff = ForceField.create(w.rc, w.ac, w.ivor, device=device)
+    rec = Receptor.create(w.coor_rec, w.rec_types - 1, device)
+    grid = DeviceGrid.build(rec, ff, np.arange(NTYPES), layout=layout, device=device, keep_plain=keep_plain,
+                            **w.grid_kw)
+    ligs = [Ligand.create(l.coor, l.types - 1, l.types - 1, device) for l in w.ligands]
+    return ScoringSystem(grid, ff, rec, flags=flags, launch=launch), ligs

@@ -1,0 +1,213 @@
+/*
+ * attract_amd.h -- C-ABI of the MI355X (gfx950) ATTRACT scoring and rigid-body
+ * minimization library (libattract_amd.so).
+ *
+ * Every buffer argument is DEVICE memory owned by the caller (hipMalloc, or a
+ * torch tensor's data pointer); the library never allocates or frees caller
+ * buffers.  All work is stream-ordered on the hipStream_t passed in (NULL = the
+ * default stream).  Every entry point returns ATJ_OK (0) or an error code; the
+ * message of the last error on the calling thread is atj_last_error().  No C++
+ * exception crosses this boundary.
+ *
+ * Conventions follow the reference scripts (sjdv1982/attract-jax):
+ *  - a pose is a 4x4 row-vector matrix M: x' = [x, y, z, 1] . M, i.e.
+ *    x'_l = sum_k x_k M[k][l] + M[3][l]                (test-grid-energrad.py:299-301)
+ *  - or a quaternion (x, y, z, w; scipy scalar-last) + translation, with
+ *    M[0:3][0:3] = as_matrix(q) (unnormalised polynomial) (test4.py:396-401)
+ *  - grid gradients are stored with the ATTRACT sign (force); the energy
+ *    gradient of the grid term is minus their trilinear interpolation
+ *    (test-grid-energrad.py:232-239).
+ */
+#ifndef ATTRACT_AMD_H
+#define ATTRACT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATJ_OK 0
+#define ATJ_ERR_INVALID 1      /* bad argument (shape, size, null pointer) */
+#define ATJ_ERR_HIP 2          /* HIP runtime error */
+#define ATJ_ERR_UNSUPPORTED 3  /* valid but unsupported configuration */
+
+/* Potential-grid memory layouts (AtjGrid.layout). */
+#define ATJ_LAYOUT_PLAIN 0   /* float [ntypes][dx][dy][dz][4]  (reference layout, read_grid.py:81) */
+#define ATJ_LAYOUT_CORNER 1  /* float [ntypes][dx-1][dy-1][dz-1][8][4]: the 8 trilinear corners of a
+                                voxel cell packed into one 128-byte record, corner c = 4*cx + 2*cy + cz */
+
+/* Flags (bitwise or). */
+#define ATJ_TERM_GRID 0x1u    /* grid trilinear term (inner, else outer grid) */
+#define ATJ_TERM_NB 0x2u      /* neighbour-list 8-6 pair term */
+#define ATJ_NB_PLATEAU 0x4u   /* subtract the plateau energy / force-shift the pair term
+                                 (test-grid-energrad.py:124-147); without it the pair term is
+                                 plain nonbon (test-nb-energrad.py:142-145) */
+#define ATJ_GRAD_EXACT 0x8u   /* gradient = exact derivative of the energy (finite-difference
+                                 checkable) instead of the ATTRACT convention */
+#define ATJ_POSE_QT 0x10u     /* poses are 7 floats (qx, qy, qz, qw, tx, ty, tz), not 16 */
+
+/* Device receptor grid.  Geometry follows read_grid.py:35-48. */
+typedef struct AtjGrid {
+  float origin[3];
+  float spacing;          /* gridspacing */
+  int32_t dim[3];         /* inner grid points */
+  int32_t dim2[3];        /* outer grid points (ignored when outer == NULL) */
+  float gridextension;
+  float plateaudis;
+  float plateaudis2;      /* plateaudis**2 rounded once to float32 (test-grid-energrad.py:113) */
+  int32_t ntypes;        /* number of potential grids (ligand grid types) */
+  int32_t layout;         /* ATJ_LAYOUT_* of inner and outer */
+  const float* inner;     /* potentials + gradients, NULL for a neighbour-only grid */
+  const float* outer;     /* NULL = no outer grid */
+  const int32_t* nbgrid;  /* [dx*dy*dz][2] = (count, start): 0-based start into nblist,
+                             start % 8 == 0 (lists are 16-byte aligned) */
+  const uint16_t* nblist; /* receptor bead indices, padded with 0xFFFF */
+} AtjGrid;
+
+/* ATTRACT pair table, compacted to receptor types x ligand types
+ * (test-grid-energrad.py:67-79).  All arrays are float32 [nrt][nlt], row-major. */
+typedef struct AtjForceField {
+  const float* rc;
+  const float* ac;
+  const float* ivor;
+  const float* emin;   /* -27 ac^4 / (256 rc^3) */
+  const float* rmin2;  /* 4 rc / (3 ac) */
+  int32_t nrt;
+  int32_t nlt;
+} AtjForceField;
+
+/* Receptor beads: float4 [nrec] = (x, y, z, ff type as int32 bit pattern). */
+typedef struct AtjReceptor {
+  const float* beads;
+  int32_t nrec;
+} AtjReceptor;
+
+/* One rigid ligand: float4 [nbeads] = (x, y, z, int32 bit pattern of
+ * grid_type | ff_type << 16).  grid_type indexes the potential grids
+ * (lig_atomtype_pos), ff_type the ligand column of the pair table. */
+typedef struct AtjLigand {
+  const float* beads;
+  int32_t nbeads;
+  int32_t reserved;
+} AtjLigand;
+
+/* A group of poses sharing one ligand (one docking job, or one ligand of a batch). */
+typedef struct AtjJob {
+  AtjLigand ligand;
+  int32_t pose_begin;  /* first pose slot of the job */
+  int32_t pose_count;
+  int32_t nsteps;      /* total ADAM steps of the job (minimize only) */
+  float lr_rot;        /* ADAM learning rate of the quaternion */
+  float lr_trans;      /* ADAM learning rate of the translation */
+  float beta1, beta2, eps;
+} AtjJob;
+
+/* Optional per-launch counters (device uint64[4]): beads inside a grid,
+ * beads inside the inner grid, listed neighbour entries (sum of K_v),
+ * evaluated pairs (dsq < plateaudis^2).  Accumulated, not reset. */
+#define ATJ_NSTATS 4
+
+/* Pose-state arrays of the minimizer (all device, indexed by pose slot). */
+typedef struct AtjPoseState {
+  float* pose;      /* [N][7] = quaternion (x, y, z, w), translation (x, y, z) */
+  float* m;         /* [N][7] ADAM first moment (quat, trans) */
+  float* v;         /* [N][7] ADAM second moment */
+  int32_t* count;   /* [N] ADAM steps done */
+  float* energy;    /* [N] energy of the last evaluated pose */
+} AtjPoseState;
+
+/* Launch configuration of the pose kernels (chosen by the autotuner). */
+typedef struct AtjLaunch {
+  int32_t waves_per_block;  /* 1..8 (0 = default 4) */
+  int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
+} AtjLaunch;
+
+const char* atj_last_error(void);
+const char* atj_version(void);
+
+/* Energy and pose gradient of N poses (reference: main + jax.value_and_grad,
+ * test-grid-energrad.py:298-317; quaternion form test4.py:396-401,415).
+ * poses: [N][16] (row-major 4x4) or [N][7] with ATJ_POSE_QT.
+ * jobs/njobs give the ligand of each pose range; pose_order (nullable) is the
+ * processing order of pose slots (a permutation of [0, N)).
+ * energy: [N]; dEdM: [N][16] = dE/dM (nullable); counter: device int32 work
+ * counter (caller-provided workspace, reset by the call); stats: nullable. */
+int atj_energrad(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                 const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
+                 const int32_t* pose_order, const float* poses, int32_t N, float* energy,
+                 float* dEdM, uint32_t flags, const AtjLaunch* launch, int32_t* counter,
+                 unsigned long long* stats, void* stream);
+
+/* Per-bead terms, for parity checks and for potential_energrad
+ * (test-grid-energrad.py:214-225) / generate_nb_table (:246-260).
+ * Outputs per (pose, bead) (all nullable):
+ *   grid4 [N][nb][4]  interpolated (E, gx, gy, gz) of the grid term (stored sign)
+ *   voxel [N][nb]     flat inner voxel index of the neighbour lookup, -1 if outside
+ *   nbene [N][nb]     neighbour-list energy of the bead
+ *   nbgrad [N][nb][3] neighbour-list dE/dx' of the bead */
+int atj_bead_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                   const AtjLigand* ligand, const float* poses, int32_t N, float* grid4,
+                   int32_t* voxel, float* nbene, float* nbgrad, uint32_t flags, void* stream);
+
+/* Per-point terms at explicit coordinates (the reference's per-bead functions take
+ * transformed coordinates all_coors_lig).  coords [M][3], types [M] = grid_type |
+ * ff_type << 16; outputs as atj_bead_terms with [M] leading dimension. */
+int atj_point_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec, const float* coords,
+                    const int32_t* types, int32_t M, float* grid4, int32_t* voxel, float* nbene, float* nbgrad,
+                    uint32_t flags, void* stream);
+
+/* Batched rigid-body ADAM minimization: every pose of every job runs
+ * min(nsteps_launch, job.nsteps - count) steps inside ONE persistent launch;
+ * pose state stays in registers between steps (reference loop: test4.py:445-455,
+ * optax.adam defaults, quaternion renormalised after every step).
+ * state.energy receives the energy evaluated in the last step. */
+int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                 const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
+                 const int32_t* pose_order, int32_t N, const AtjPoseState* state,
+                 int32_t nsteps_launch, uint32_t flags, const AtjLaunch* launch,
+                 int32_t* counter, unsigned long long* stats, void* stream);
+
+/* One ADAM step of N poses from dE/dM (the plain one-launch-per-step form):
+ * quaternion chain rule through as_matrix, optax adam update, renormalisation. */
+int atj_adam_step(const float* dEdM, int32_t N, const AtjPoseState* state, float lr_rot,
+                  float lr_trans, float beta1, float beta2, float eps, void* stream);
+
+/* Adjoint of the pair term w.r.t. the pair table: for weights w[N]
+ * (upstream dL/dE_pose) accumulates sum_pose w * dE_pose/d(rc, ac, emin) into
+ * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated) and
+ * also writes energy/dEdM like atj_energrad (recomputation, no stored pairs).
+ * Reduce the slabs with atj_reduce_partials. Requires blocks = launch->blocks > 0. */
+int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                         const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
+                         const float* poses, int32_t N, const float* weights, float* energy,
+                         float* dEdM, float* partial, uint32_t flags, const AtjLaunch* launch,
+                         int32_t* counter, void* stream);
+
+/* out[i] = sum_b partial[b][i] for i < n, in block order (deterministic). */
+int atj_reduce_partials(const float* partial, int32_t blocks, int32_t n, float* out, void* stream);
+
+/* Grid construction on the device (semantics of test-calc-grid-energy.py:132-196
+ * plus a force channel):
+ * potentials of ligand grid type t at every inner / outer grid point, plain layout.
+ * lig_fftype[ntypes] gives the pair-table column of each grid type. */
+int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff,
+                              const AtjReceptor* rec, const int32_t* lig_fftype_dev,
+                              float cutoff, float* inner_out, float* outer_out, void* stream);
+/* Neighbour lists: receptor beads with |p - r| < neighbourdis of every inner grid point p.
+ * Pass 1 writes counts [dx*dy*dz]; the caller turns round-up-to-8 counts into 0-based
+ * starts; pass 2 writes nbgrid [V][2] and nblist (ascending receptor index). */
+int atj_grid_count_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis,
+                              int32_t* counts, void* stream);
+int atj_grid_fill_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis,
+                             const int32_t* counts, const int64_t* starts, int32_t* nbgrid,
+                             uint16_t* nblist, void* stream);
+/* Plain [t][x][y][z][4] -> corner-packed [t][x-1][y-1][z-1][8][4]. */
+int atj_grid_pack_corners(const float* plain, int32_t ntypes, const int32_t dim[3],
+                          float* packed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATTRACT_AMD_H */

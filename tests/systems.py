@@ -1,0 +1,82 @@
+"""Small synthetic docking systems for the tests, built entirely by the oracle (CPU)."""
+from collections import namedtuple
+
+import numpy as np
+
+from oracle import oracle as O
+
+
+def tiny_table(rng, ntypes=4):
+    rmin = rng.uniform(3.0, 5.0, size=(ntypes, ntypes))
+    rmin = (rmin + rmin.T) / 2
+    eabs = rng.uniform(0.1, 1.5, size=(ntypes, ntypes))
+    eabs = (eabs + eabs.T) / 2
+    ivor = np.where(rng.random((ntypes, ntypes)) < 0.25, -1.0, 1.0)
+    ivor = np.triu(ivor) + np.triu(ivor, 1).T
+    rbc = rmin * np.sqrt(3) / 2
+    abc = 256 / 27 * eabs
+    return O.ff_from_par(rbc, abc, ivor)
+
+
+def build_reference_grid(coor_rec, rec_types, ff, ntypes, origin, spacing, dim, dim2, gridextension, plateaudis,
+                         neighbourdis, cutoff=50.0, with_potentials=True):
+    """A reference-format (read_grid, CSR) Grid built by the oracle's grid semantics."""
+    origin = np.asarray(origin, np.float64)
+    ii = np.stack(np.meshgrid(*[np.arange(d) for d in dim], indexing="ij"), -1).reshape(-1, 3)
+    pts = origin + spacing * ii
+    d = {}
+    d["gridspacing"] = float(spacing)
+    d["gridextension"] = int(gridextension)
+    d["plateaudis"] = float(plateaudis)
+    d["neighbourdis"] = float(neighbourdis)
+    alphabet = np.zeros(99, bool)
+    alphabet[:ntypes] = True
+    d["alphabet"] = alphabet
+    d["origin"] = origin
+    d["dim"] = np.array(dim)
+    d["dim2"] = np.array(dim2)
+    d["natoms"] = len(coor_rec)
+    lists = [O.grid_point_neighbours(p, coor_rec, neighbourdis) for p in pts]
+    counts = np.array([len(l) for l in lists])
+    d["neighbours"] = np.concatenate(lists).astype(np.uint32) if counts.sum() else np.zeros(0, np.uint32)
+    if with_potentials:
+        inner = np.stack([O.grid_point_potential(pts, coor_rec, rec_types, t, ff, plateaudis, cutoff)
+                          for t in range(ntypes)]).reshape(ntypes, *dim, 4)
+        d["inner_potential_grid"] = inner.astype(np.float32)
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    ng = np.stack([counts, np.where(counts > 0, starts + 1, 0)], -1).astype(np.int32).reshape(*dim, 2)
+    d["neighbour_grid"] = ng
+    if with_potentials:
+        jj = np.stack(np.meshgrid(*[np.arange(v) for v in dim2], indexing="ij"), -1).reshape(-1, 3)
+        pts2 = origin + spacing * (2 * jj - gridextension)
+        outer = np.stack([O.grid_point_potential(pts2, coor_rec, rec_types, t, ff, plateaudis, cutoff)
+                          for t in range(ntypes)]).reshape(ntypes, *dim2, 4)
+        d["outer_potential_grid"] = outer.astype(np.float32)
+    return namedtuple("Grid", d.keys())(*d.values())
+
+
+class TinySystem:
+    """40-bead receptor in a 6 A sphere, 4 types, inner 14^3 @1 A, outer 10^3 @2 A."""
+
+    def __init__(self, seed=0, nlig=5, npose=64, with_potentials=True, plateau=3.0):
+        rng = np.random.default_rng(seed)
+        from attract_jax_amd.workloads import random_chain, random_receptor
+        self.coor_rec = random_receptor(40, 6.0, 2.0, rng)
+        self.rec_types = rng.integers(0, 4, size=40)
+        self.ff = tiny_table(rng)
+        self.ntypes = 4
+        self.spacing = 1.0
+        nbdis = plateau + self.spacing * np.sqrt(3) / 2 + 1e-3
+        self.grid = build_reference_grid(self.coor_rec, self.rec_types, self.ff, 4, origin=(-7.0, -7.0, -7.0),
+                                         spacing=1.0, dim=(14, 14, 14), dim2=(10, 10, 10), gridextension=6,
+                                         plateaudis=plateau, neighbourdis=nbdis, with_potentials=with_potentials)
+        self.coor_lig = random_chain(nlig, rng, bond=3.0, min_sep=2.5)
+        self.lig_types = rng.integers(0, 4, size=nlig)
+        from attract_jax_amd.pose import random_poses, mat_from_qt
+        q, t = random_poses(npose, rng, 3.0, 9.0)
+        self.quat, self.trans = q, t
+        self.mat = mat_from_qt(q, t)
+
+    def oracle_energrad(self, mat=None, **kw):
+        return O.energrad(self.mat if mat is None else mat, self.coor_lig, self.lig_types, self.lig_types,
+                          self.coor_rec, self.rec_types, self.ff, self.grid, **kw)

@@ -1,0 +1,292 @@
+"""HIP kernels (through the C-ABI) against the oracle: the FP32 op-order-pinned CPU twin
+(bit-exact voxel indices and pair membership; energies / gradients within 1e-4
+relative) and the FP64 NumPy oracle (within 1e-4 relative)."""
+import numpy as np
+import pytest
+import torch
+
+import attract_jax_amd as A
+from attract_jax_amd import _lib
+from attract_jax_amd.minimize import BatchPlan, PoseState, minimize, minimize_plain
+from attract_jax_amd.pose import mat_from_qt
+from attract_jax_amd.workloads import build_system, c1, c2, c3
+from oracle import oracle as O
+from oracle.twin import T_EXACT, T_GRID, T_NB, T_PLATEAU, Twin
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _ff_arrays(system):
+    f = system.ff
+    return [t.cpu().numpy() for t in (f.rc, f.ac, f.ivor, f.emin, f.rmin2)]
+
+
+class Sys:
+    def __init__(self, w, layout=_lib.ATJ_LAYOUT_CORNER):
+        assert torch.cuda.is_available()
+        self.w = w
+        self.system, self.ligs = build_system(w, DEV, layout=layout, keep_plain=True)
+        self.grid = self.system.grid.to_reference_grid()
+        self.ff = O.FF(*[a.astype(np.float64) for a in _ff_arrays(self.system)])
+        self.twins = [Twin(self.grid, _ff_arrays(self.system), w.coor_rec, w.rec_types - 1, l.coor, l.types - 1,
+                           l.types - 1) for l in w.ligands]
+
+    def mats(self, sl=slice(None)):
+        return mat_from_qt(self.w.quat[sl], self.w.trans[sl]).astype(np.float32)
+
+    def oracle(self, mat, k=0, **kw):
+        l = self.w.ligands[k]
+        return O.energrad(mat.astype(np.float64), l.coor, l.types - 1, l.types - 1, self.w.coor_rec,
+                          self.w.rec_types - 1, self.ff, self.grid, **kw)
+
+
+@pytest.fixture(scope="module")
+def s1():
+    return Sys(c1())
+
+
+@pytest.fixture(scope="module")
+def s2():
+    return Sys(c2(npose=2048))
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def _grel(G, Gref):
+    return np.abs(G - Gref) / np.maximum(np.abs(Gref).max(axis=(1, 2), keepdims=True), 1.0)
+
+
+def test_grid_builder_vs_oracle(s1):
+    rng = np.random.default_rng(0)
+    g = s1.grid
+    idx = np.stack([rng.integers(0, d, 300) for d in g.dim], 1)
+    pts = g.origin + g.gridspacing * idx
+    for t in (0, 7, 31):
+        ref = O.grid_point_potential(pts, s1.w.coor_rec, s1.w.rec_types - 1, t, s1.ff, g.plateaudis)
+        got = g.inner_potential_grid[t][idx[:, 0], idx[:, 1], idx[:, 2]]
+        scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+        assert (np.abs(got - ref) / scale).max() < 1e-4
+    idx2 = np.stack([rng.integers(0, d, 100) for d in g.dim2], 1)
+    pts2 = g.origin + g.gridspacing * (2 * idx2 - g.gridextension)
+    ref = O.grid_point_potential(pts2, s1.w.coor_rec, s1.w.rec_types - 1, 3, s1.ff, g.plateaudis)
+    got = g.outer_potential_grid[3][idx2[:, 0], idx2[:, 1], idx2[:, 2]]
+    assert (np.abs(got - ref) / np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)).max() < 1e-4
+    counts, starts, lists = O.csr_lists(g)
+    flat = np.ravel_multi_index(idx.T, g.dim)
+    for v, p in zip(flat, pts):
+        ref = O.grid_point_neighbours(p, s1.w.coor_rec, g.neighbourdis)
+        got = lists[starts[v]:starts[v] + counts[v]]
+        if not np.array_equal(got, ref):  # only beads at the FP32 rounding edge may differ
+            dd = np.linalg.norm(s1.w.coor_rec[np.setxor1d(got, ref).astype(int)] - p, axis=1)
+            assert np.all(np.abs(dd - g.neighbourdis) < 1e-4)
+
+
+@pytest.mark.parametrize("layout", [_lib.ATJ_LAYOUT_CORNER, _lib.ATJ_LAYOUT_PLAIN])
+def test_energrad_vs_twin_and_oracle(s1, layout):
+    system = s1.system
+    if layout != system.grid.layout:
+        system, ligs = build_system(s1.w, DEV, layout=layout)
+    else:
+        ligs = s1.ligs
+    mat = s1.mats()
+    E, G = system.energrad(mat, ligs[0], stats=True)
+    E, G = E.cpu().numpy(), G.cpu().numpy()
+    Et, Gt = s1.twins[0].energrad(mat)
+    assert _rel(E, Et).max() < 1e-4
+    assert _grel(G, Gt).max() < 1e-4
+    Eo, Go = s1.oracle(mat)
+    assert _rel(E, Eo).max() < 1e-4
+    assert _grel(G, Go).max() < 1e-4
+    assert np.count_nonzero(E) > len(E) // 2
+
+
+def test_layouts_bit_identical(s1):
+    sp, lp = build_system(s1.w, DEV, layout=_lib.ATJ_LAYOUT_PLAIN)
+    mat = s1.mats()
+    E1, G1 = s1.system.energrad(mat, s1.ligs[0])
+    E2, G2 = sp.energrad(mat, lp[0])
+    assert torch.equal(E1, E2) and torch.equal(G1, G2)
+
+
+def test_voxels_and_pairs_bit_exact(s1):
+    mat = s1.mats()
+    grid4, voxel, nbe, nbg = s1.system.bead_terms(mat, s1.ligs[0])
+    out = s1.twins[0].bead_terms(mat)
+    assert np.array_equal(voxel.cpu().numpy(), out["voxel"])
+    assert (out["voxel"] >= 0).any()
+    assert np.array_equal(grid4.cpu().numpy(), out["grid4"])  # same fma order in both
+    assert _rel(nbe.cpu().numpy(), out["nbene"]).max() < 1e-4
+
+
+def test_stats_counters(s1):
+    s1.system.stats.zero_()
+    mat = s1.mats()
+    s1.system.energrad(mat, s1.ligs[0], stats=True)
+    st = s1.system.stats.cpu().numpy()
+    out = s1.twins[0].bead_terms(mat)
+    assert st[1] == (out["voxel"] >= 0).sum()
+    assert st[3] == out["npairs"].sum()
+
+
+def test_exact_gradient_vs_fp64_fd(s1):
+    flags = _lib.ATJ_TERM_GRID | _lib.ATJ_TERM_NB | _lib.ATJ_NB_PLATEAU | _lib.ATJ_GRAD_EXACT
+    mat = s1.mats(slice(0, 64))
+    E, G = s1.system.energrad(mat, s1.ligs[0], flags=flags)
+    G = G.cpu().numpy()
+    h = 1e-4
+    fd = np.zeros_like(G, dtype=np.float64)
+    for k in range(4):
+        for l in range(3):
+            mp, mm = mat.astype(np.float64), mat.astype(np.float64)
+            mp[:, k, l] += h
+            mm[:, k, l] -= h
+            fd[:, k, l] = (s1.oracle(mp, exact=True)[0] - s1.oracle(mm, exact=True)[0]) / (2 * h)
+    ok = np.all(_grel(G, fd) < 1e-3, axis=(1, 2))
+    assert ok.mean() >= 0.9
+
+
+def test_reference_named_api(s1):
+    """potential_energrad / potential_energy / neighbour_energy / main / value_and_grad_main
+    on reference-style host inputs (test-grid-energrad.py call surface)."""
+    w, g = s1.w, s1.grid
+    lig = w.ligands[0]
+    mat = s1.mats(slice(0, 200))
+    xp = O.transform(mat.astype(np.float64), lig.coor).astype(np.float32)
+    pe = A.potential_energrad(xp, lig.types - 1, g).cpu().numpy()
+    raw, _, _ = O.grid_energrad(xp.astype(np.float64), lig.types - 1, g)
+    assert (np.abs(pe - raw) / np.maximum(np.abs(raw).max(-1, keepdims=True), 1)).max() < 1e-4
+    pot = A.potential_energy(xp, lig.types - 1, g).cpu().numpy()
+    assert _rel(pot, raw[..., 0].sum(1)).max() < 1e-4
+    nb = A.neighbour_energy(w.coor_rec, w.rec_types - 1, xp, lig.types - 1, s1.ff, g).cpu().numpy()
+    En, _, _ = O.nb_energrad(xp.astype(np.float64), lig.types - 1, w.coor_rec, w.rec_types - 1, s1.ff, g, True)
+    assert _rel(nb, En.sum(1)).max() < 1e-4
+    (tot, E), G = A.value_and_grad_main(mat, w.coor_rec, w.rec_types - 1, lig.coor, lig.types - 1, lig.types - 1,
+                                        s1.ff, g)
+    Eo, Go = s1.oracle(mat)
+    assert _rel(E.cpu().numpy(), Eo).max() < 1e-4
+    assert _grel(G.cpu().numpy(), Go).max() < 1e-4
+    q = w.quat[:200].astype(np.float32)
+    (tq, Eq), (gq, gt) = A.value_and_grad_quat(q, w.trans[:200], w.coor_rec, w.rec_types - 1, lig.coor,
+                                                lig.types - 1, lig.types - 1, s1.ff, g)
+    M = O.mat_from_qt(q.astype(np.float64), w.trans[:200])
+    Eo, Go = s1.oracle(M)
+    assert _rel(Eq.cpu().numpy(), Eo).max() < 1e-4
+    gq_o = O.quat_grad(q.astype(np.float64), Go[:, :3, :3])
+    assert (np.abs(gq.cpu().numpy() - gq_o) / np.maximum(np.abs(gq_o).max(1, keepdims=True), 1)).max() < 1e-3
+
+
+def test_minimize_vs_twin_and_oracle(s1):
+    w = s1.w
+    plan = BatchPlan(s1.system, s1.ligs, [w.npose], w.nsteps, w.lr_rot, w.lr_trans)
+    st = PoseState.create(w.quat, w.trans, DEV)
+    minimize(s1.system, plan, st, w.nsteps)
+    P = st.pose.cpu().numpy()
+    assert np.all(st.count.cpu().numpy() == w.nsteps)
+    Pt, Et = s1.twins[0].minimize(np.concatenate([w.quat, w.trans], 1), w.nsteps, w.lr_rot, w.lr_trans)
+    lig = w.ligands[0]
+    xg = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), lig.coor)
+    xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
+    r = O.rmsd(xg, xt)
+    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
+    Eg = st.energy.cpu().numpy()
+    close = r < 1e-3
+    assert _rel(Eg[close], Et[close]).max() < 1e-3
+
+    sub = slice(0, 200)
+
+    def fn(q, t):
+        return s1.oracle(O.mat_from_qt(q, t))
+
+    qo, to, Eo = O.adam_minimize(w.quat[sub], w.trans[sub], w.nsteps, w.lr_rot, w.lr_trans, fn)
+    xo = O.transform(O.mat_from_qt(qo, to), lig.coor)
+    r64 = O.rmsd(xg[sub], xo)
+    assert np.median(r64) < 0.05 and (r64 < 0.05).mean() >= 0.8
+
+
+def test_batched_equals_plain(s1):
+    w = s1.w
+    n = 300
+    plan = BatchPlan(s1.system, s1.ligs, [n], 25, w.lr_rot, w.lr_trans)
+    a = PoseState.create(w.quat[:n], w.trans[:n], DEV)
+    minimize(s1.system, plan, a, 25)
+    b = PoseState.create(w.quat[:n], w.trans[:n], DEV)
+    minimize_plain(s1.system, s1.ligs[0], b, 25, w.lr_rot, w.lr_trans)
+    assert torch.equal(a.pose, b.pose)
+    assert torch.equal(a.energy, b.energy)
+
+
+def test_chunked_launches_equal_single_launch(s1):
+    w = s1.w
+    n = 256
+    plan = BatchPlan(s1.system, s1.ligs, [n], 30, w.lr_rot, w.lr_trans)
+    a = PoseState.create(w.quat[:n], w.trans[:n], DEV)
+    minimize(s1.system, plan, a, 30)
+    b = PoseState.create(w.quat[:n], w.trans[:n], DEV)
+    for _ in range(3):
+        minimize(s1.system, plan, b, 10)
+    minimize(s1.system, plan, b, 10)  # job already finished: no-op
+    assert torch.equal(a.pose, b.pose) and torch.equal(b.count, a.count)
+
+
+def test_uneven_ligands_mix_invariance_and_parity():
+    w = c3(nligands=6, poses_per_ligand=64, nsteps=20)
+    w.ligands[1].coor = w.ligands[1].coor[:1]  # a one-bead ligand
+    w.ligands[1].types = w.ligands[1].types[:1]
+    from attract_jax_amd.workloads import LigandSpec, random_chain
+    rng = np.random.default_rng(5)
+    w.ligands[2] = LigandSpec(random_chain(150, rng), rng.integers(1, 33, 150))  # > 64 beads: chunked
+    system, ligs = build_system(w, DEV, keep_plain=True)
+    grid = system.grid.to_reference_grid()
+    plan = BatchPlan(system, ligs, w.pose_counts, w.nsteps, w.lr_rot, w.lr_trans)
+    st = PoseState.create(w.quat, w.trans, DEV)
+    minimize(system, plan, st, w.nsteps)
+    for k in range(len(ligs)):
+        sl = slice(plan.offsets[k], plan.offsets[k + 1])
+        alone_plan = BatchPlan(system, [ligs[k]], [w.pose_counts[k]], w.nsteps, w.lr_rot, w.lr_trans)
+        alone = PoseState.create(w.quat[sl], w.trans[sl], DEV)
+        minimize(system, alone_plan, alone, w.nsteps)
+        assert torch.equal(alone.pose, st.pose[sl]) and torch.equal(alone.energy, st.energy[sl])
+        l = w.ligands[k]
+        tw = Twin(grid, _ff_arrays(system), w.coor_rec, w.rec_types - 1, l.coor, l.types - 1, l.types - 1)
+        mat = mat_from_qt(w.quat[sl], w.trans[sl]).astype(np.float32)
+        E, G = system.energrad(mat, ligs[k])
+        Et, Gt = tw.energrad(mat)
+        assert _rel(E.cpu().numpy(), Et).max() < 1e-4
+        assert _grel(G.cpu().numpy(), Gt).max() < 1e-4
+
+
+def test_c2_scale_subset(s2):
+    mat = s2.mats()
+    s2.system.stats.zero_()
+    E, G = s2.system.energrad(mat, s2.ligs[0], stats=True)
+    Et, Gt = s2.twins[0].energrad(mat)
+    assert _rel(E.cpu().numpy(), Et).max() < 1e-4
+    assert _grel(G.cpu().numpy(), Gt).max() < 1e-4
+    Eo, Go = s2.oracle(mat[:256])
+    assert _rel(E.cpu().numpy()[:256], Eo).max() < 1e-4
+    st = s2.system.stats.cpu().numpy()
+    assert st[3] > 0 and st[1] > 0
+    vox = s2.system.bead_terms(mat[:512], s2.ligs[0])[1].cpu().numpy()
+    assert np.array_equal(vox, s2.twins[0].bead_terms(mat[:512])["voxel"])
+
+
+def test_edge_cases(s1):
+    w = s1.w
+    E, G = s1.system.energrad(np.zeros((0, 4, 4), np.float32), s1.ligs[0])
+    assert E.numel() == 0
+    far = s1.mats(slice(0, 8)).copy()
+    far[:, 3, :3] = 1000.0
+    E, G = s1.system.energrad(far, s1.ligs[0])
+    assert torch.count_nonzero(E) == 0 and torch.count_nonzero(G) == 0
+    with pytest.raises(_lib.AttractError):
+        bad = _lib.AtjLaunch(9, 0)
+        old = s1.system.launch
+        s1.system.launch = bad
+        try:
+            s1.system.energrad(s1.mats(slice(0, 4)), s1.ligs[0])
+        finally:
+            s1.system.launch = old
