@@ -10,7 +10,7 @@ import torch
 from . import _lib
 from .minimize import BatchPlan, PoseState, minimize
 
-CANDIDATES = (2, 4, 8)
+CANDIDATES = (4, 8, 16)
 _cache = {}
 
 

@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kWave = 64;
 constexpr int kSeg = 8;            // receptor indices per 16-byte list segment
-constexpr int kMaxWavesPerBlock = 8;
+constexpr int kMaxWavesPerBlock = 16;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
 
 enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2 };
@@ -65,10 +65,31 @@ __device__ __forceinline__ float uni(float v) {
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Value of lane `l` (uniform result in a scalar register).
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// DPP move with zero for lanes outside row_mask.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dppi(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Sum over the 64 lanes in a fixed order (quad, half-row and row mirrors, then the
+// row broadcasts of lanes 15 and 31); the total is returned uniformly from lane 63.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
-  return v;
+  v += dpp<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);       // row_half_mirror
+  v += dpp<0x140>(v);       // row_mirror
+  v += dpp<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
+  v += dpp<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
+  return lane_value(v, 63);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
@@ -77,12 +98,14 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
   return v;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    int t = __shfl_up(v, off, kWave);
-    if (lane >= off) v += t;
-  }
+// Inclusive prefix sum over the 64 lanes (row shifts, then row broadcasts).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += dppi<0x111>(v);       // row_shr:1
+  v += dppi<0x112>(v);       // row_shr:2
+  v += dppi<0x114>(v);       // row_shr:4
+  v += dppi<0x118>(v);       // row_shr:8
+  v += dppi<0x142, 0xA>(v);  // row_bcast:15
+  v += dppi<0x143, 0xC>(v);  // row_bcast:31
   return v;
 }
 
@@ -293,15 +316,15 @@ struct KArgs {
   unsigned flags;
   int* counter;
   unsigned long long* stats;
-  int rec_in_lds;
 };
 
 // Per-pose energy and dE/dM for a uniform pose matrix M; acc[] returns the
 // wave-reduced sums on every lane.
-template <int MODE>
+template <int MODE, bool REC_LDS>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               float* s_pgrad, WaveScratch* ws, const float4* __restrict__ lig, int nb,
                               const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
+  const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
   float M[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
@@ -352,8 +375,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     if (!do_nb) continue;
     const int nseg = (cnt + kSeg - 1) / kSeg;
-    const int incl = wave_incl_scan(nseg, lane);
-    const int total = __shfl(incl, kWave - 1, kWave);
+    const int incl = wave_incl_scan(nseg);
+    const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
     if (total == 0) continue;
     ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
     ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
@@ -363,73 +386,82 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     int jb = 0;
     for (int base = 0; base < total; base += kWave) {
       const int item = base + lane;
-      if (item < total) {
+      const bool act = item < total;
+      if (act)
         while (ws->segincl[jb] <= item) ++jb;
-        const int sbase = jb ? ws->segincl[jb - 1] : 0;
-        const int seg = item - sbase;
-        const float4 bx = ws->xp[jb];
-        const float4 bl = ws->xl[jb];
-        const int bcnt = ws->cnt[jb];
-        const int bstart = __float_as_int(bl.w);
-        const int blt = __float_as_int(bx.w);
-        const int k0 = seg * kSeg;
-        const int n = min(kSeg, bcnt - k0);
-        float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-        unsigned npair = 0;
-        // two halves of 4 entries (8 bytes each) keep the number of loads in flight bounded
-#pragma unroll 1
-        for (int h = 0; h < 2; ++h) {
-          const uint2 ent = *reinterpret_cast<const uint2*>(G.nblist + bstart + k0 + 4 * h);
+      const int sbase = jb ? ws->segincl[jb - 1] : 0;
+      const float4 bx = ws->xp[jb];
+      const float4 bl = ws->xl[jb];
+      const int bcnt = ws->cnt[jb];
+      const int bstart = __float_as_int(bl.w);
+      const unsigned blt = (unsigned)__float_as_int(bx.w);
+      const int k0 = act ? (item - sbase) * kSeg : 0;
+      const int n = act ? min(kSeg, bcnt - k0) : 0;
+      const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + (act ? bstart + k0 : 0));
+      const unsigned words[4] = {ent.x, ent.y, ent.z, ent.w};
+      float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+      // branch-free batches of 4 neighbours: independent LDS reads, masked results
 #pragma unroll
-          for (int cc = 0; cc < 4; ++cc) {
-            if (4 * h + cc < n) {
-              const unsigned wd = (cc < 2) ? ent.x : ent.y;
-              const unsigned r = (cc & 1) ? (wd >> 16) : (wd & 0xffffu);
-              const float4 rb = a.rec_in_lds ? s_rec[r] : a.rec[r];
-              const float ddx = bx.x - rb.x, ddy = bx.y - rb.y, ddz = bx.z - rb.z;
-              const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-              if (dsq < P2) {
-                const int rt = __float_as_int(rb.w);
-                const int pi = rt * a.nlt + blt;
-                const float4 pa = s_par[2 * pi];
-                const float4 pb = s_par[2 * pi + 1];
-                float E, gs;
-                pair_term(dsq, pa, pb, plateau, exact, E, gs);
-                e += E;
-                gx = fmaf(gs, ddx, gx);
-                gy = fmaf(gs, ddy, gy);
-                gz = fmaf(gs, ddz, gz);
-                ++npair;
-                if (MODE == kAdjoint) {
-                  float d[3];
-                  pair_param_grad(dsq, pa, d);
-                  if (plateau) {
-                    float dp[3];
-                    pair_param_grad(P2, pa, dp);
-                    d[0] -= dp[0];
-                    d[1] -= dp[1];
-                    d[2] -= dp[2];
-                  }
-                  const int ntp = a.nrt * a.nlt;
-                  atomicAdd(&s_pgrad[pi], wgt * d[0]);
-                  atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
-                  atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
-                }
-              }
+      for (int h = 0; h < 2; ++h) {
+        float4 rb[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+          const unsigned wd = words[2 * h + (cc >> 1)];
+          const unsigned r = (cc & 1) ? (wd >> 16) : (wd & 0xffffu);
+          rb[cc] = recp[(4 * h + cc < n) ? r : 0u];
+        }
+        float dsq[4], dd[4][3];
+        bool ok[4];
+        float4 pa[4], pb[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+          dd[cc][0] = bx.x - rb[cc].x;
+          dd[cc][1] = bx.y - rb[cc].y;
+          dd[cc][2] = bx.z - rb[cc].z;
+          dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
+          ok[cc] = (4 * h + cc < n) && (dsq[cc] < P2);
+          const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * (unsigned)a.nlt + blt : 0u;
+          pa[cc] = s_par[2 * pi];
+          pb[cc] = s_par[2 * pi + 1];
+        }
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+          float E, gs;
+          pair_term(dsq[cc], pa[cc], pb[cc], plateau, exact, E, gs);
+          E = ok[cc] ? E : 0.f;
+          gs = ok[cc] ? gs : 0.f;
+          e += E;
+          gx = fmaf(gs, dd[cc][0], gx);
+          gy = fmaf(gs, dd[cc][1], gy);
+          gz = fmaf(gs, dd[cc][2], gz);
+          st[3] += ok[cc] ? 1u : 0u;
+          if (MODE == kAdjoint && ok[cc]) {
+            float d[3];
+            pair_param_grad(dsq[cc], pa[cc], d);
+            if (plateau) {
+              float dp[3];
+              pair_param_grad(P2, pa[cc], dp);
+              d[0] -= dp[0];
+              d[1] -= dp[1];
+              d[2] -= dp[2];
             }
+            const unsigned ntp = (unsigned)(a.nrt * a.nlt);
+            const unsigned pi = (unsigned)__float_as_int(rb[cc].w) * (unsigned)a.nlt + blt;
+            atomicAdd(&s_pgrad[pi], wgt * d[0]);
+            atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
+            atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
           }
         }
-        st[3] += npair;
-        acc[0] += e;
-        const float g[3] = {gx, gy, gz};
-        const float xb[3] = {bl.x, bl.y, bl.z};
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
-#pragma unroll
-        for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
       }
+      acc[0] += e;
+      const float g[3] = {gx, gy, gz};
+      const float xb[3] = {bl.x, bl.y, bl.z};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
     }
     wave_sync();
   }
@@ -455,7 +487,7 @@ __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float 
 // Stage the receptor beads and the derived pair table into LDS.
 __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s_pgrad) {
   const int tid = threadIdx.x;
-  if (a.rec_in_lds)
+  if (s_rec)
     for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = a.rec[i];
   const int ntp = a.nrt * a.nlt;
   const float P2 = a.G.p2;
@@ -478,7 +510,7 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
-template <int MODE>
+template <int MODE, bool REC_LDS>
 __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
   const int nwaves = blockDim.x / kWave;
@@ -487,10 +519,10 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int ntp = a.nrt * a.nlt;
   float4* s_par = smem;
   float4* s_rec = s_par + 2 * ntp;
-  const int nrec_lds = a.rec_in_lds ? a.nrec : 0;
+  const int nrec_lds = REC_LDS ? a.nrec : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
-  stage_lds(a, s_rec, s_par, s_pgrad);
+  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_pgrad);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
   const bool qt = (a.flags & ATJ_POSE_QT) != 0;
@@ -499,7 +531,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   for (;;) {
     int k = 0;
     if (lane == 0) k = atomicAdd(a.counter, 1);
-    k = __shfl(k, 0, kWave);
+    k = __builtin_amdgcn_readfirstlane(k);
     if (k >= a.N) break;
     const int p = a.pose_order ? a.pose_order[k] : k;
     const int jid = a.pose_job ? a.pose_job[p] : 0;
@@ -508,72 +540,61 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     const int nb = job.ligand.nbeads;
     float acc[kAcc];
     if (MODE == kMinimize) {
-      float q[4], t[3], m[7], v[7];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) q[i] = a.pose7[7 * p + i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) t[i] = a.pose7[7 * p + 4 + i];
-#pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        m[i] = a.mom1[7 * p + i];
-        v[i] = a.mom2[7 * p + i];
-      }
-      int cnt = a.count[p];
+      // Lane i < 7 owns parameter i of (qx, qy, qz, qw, tx, ty, tz) and its ADAM moments;
+      // the pose itself is uniform (scalar registers) on every lane.
+      const int pi = lane < 7 ? lane : 6;
+      float prm = a.pose7[7 * p + pi];
+      float m1 = a.mom1[7 * p + pi];
+      float m2 = a.mom2[7 * p + pi];
+      int cnt = uni(a.count[p]);
       float elast = a.elast[p];
+      const float lr = lane < 4 ? job.lr_rot : job.lr_trans;
+      const float b1 = job.beta1, b2 = job.beta2;
+      float q[4], t[3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = lane_value(prm, i);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) t[i] = lane_value(prm, 4 + i);
       const int nst = min(a.nsteps_launch, job.nsteps - cnt);
       for (int s = 0; s < nst; ++s) {
         float R[9];
         quat_to_mat(q, R);
         const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
                              R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
-        pose_energrad<MODE>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
+        pose_energrad<MODE, REC_LDS>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
         elast = acc[0];
         float A[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) A[i] = acc[1 + i];
-        float gr[7];
-        quat_grad(q, A, gr);
-        gr[4] = acc[10];
-        gr[5] = acc[11];
-        gr[6] = acc[12];
+        float gq[4];
+        quat_grad(q, A, gq);
+        const float g = lane == 0 ? gq[0] : lane == 1 ? gq[1] : lane == 2 ? gq[2] : lane == 3 ? gq[3]
+                      : lane == 4 ? acc[10] : lane == 5 ? acc[11] : acc[12];
         ++cnt;
         // optax.adam: scale_by_adam (bias-corrected, eps outside the sqrt) then -lr
-        const float b1 = job.beta1, b2 = job.beta2;
         const float bc1 = 1.f - powf(b1, (float)cnt);
         const float bc2 = 1.f - powf(b2, (float)cnt);
-        float prm[7] = {q[0], q[1], q[2], q[3], t[0], t[1], t[2]};
+        m1 = (1.f - b1) * g + b1 * m1;
+        m2 = (1.f - b2) * (g * g) + b2 * m2;
+        const float upd = (m1 / bc1) / (sqrtf(m2 / bc2) + job.eps);
+        prm = prm + (-lr) * upd;
+        // Rotation.from_quat(as_quat()): renormalise the quaternion
+        float qn[4];
 #pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          m[i] = (1.f - b1) * gr[i] + b1 * m[i];
-          v[i] = (1.f - b2) * (gr[i] * gr[i]) + b2 * v[i];
-          const float mh = m[i] / bc1;
-          const float vh = v[i] / bc2;
-          const float upd = mh / (sqrtf(vh) + job.eps);
-          const float lr = i < 4 ? job.lr_rot : job.lr_trans;
-          prm[i] = prm[i] + (-lr) * upd;
-        }
-        // Rotation.from_quat(as_quat()): renormalise
-        const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+        for (int i = 0; i < 4; ++i) qn[i] = lane_value(prm, i);
+        const float nrm = sqrtf(fmaf(qn[3], qn[3], fmaf(qn[2], qn[2], fmaf(qn[1], qn[1], qn[0] * qn[0]))));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = uni(prm[i] / nrm);
+        for (int i = 0; i < 4; ++i) q[i] = uni(qn[i] / nrm);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) t[i] = uni(prm[4 + i]);
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          m[i] = uni(m[i]);
-          v[i] = uni(v[i]);
-        }
+        for (int i = 0; i < 3; ++i) t[i] = lane_value(prm, 4 + i);
+        prm = lane == 0 ? q[0] : lane == 1 ? q[1] : lane == 2 ? q[2] : lane == 3 ? q[3] : prm;
+      }
+      if (lane < 7) {
+        a.pose7[7 * p + lane] = prm;
+        a.mom1[7 * p + lane] = m1;
+        a.mom2[7 * p + lane] = m2;
       }
       if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a.pose7[7 * p + i] = q[i];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) a.pose7[7 * p + 4 + i] = t[i];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          a.mom1[7 * p + i] = m[i];
-          a.mom2[7 * p + i] = v[i];
-        }
         a.count[p] = cnt;
         a.elast[p] = elast;
       }
@@ -581,7 +602,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       float M[16];
       load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
       const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      pose_energrad<MODE>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
+      pose_energrad<MODE, REC_LDS>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
       if (lane == 0) {
         a.energy[p] = acc[0];
         if (a.dEdM) {
@@ -934,6 +955,27 @@ int device_info(DevInfo& d) {
   return ATJ_OK;
 }
 
+template <int MODE, bool REC_LDS>
+int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
+  auto kern = pose_kernel<MODE, REC_LDS>;
+  if (lds > 65536)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  if (blocks == 0) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
+    if (per_cu < 1) per_cu = 1;
+    blocks = per_cu * d.cus;
+    const long long need = ((long long)a.N + wpb - 1) / wpb;
+    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
+  }
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(wpb * kWave), lds, stream, a);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
 template <int MODE>
 int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (!a.counter) return fail(ATJ_ERR_INVALID, "work counter is NULL");
@@ -942,32 +984,14 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   int rc = device_info(d);
   if (rc) return rc;
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
-  if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 8");
+  if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   const size_t ntp = (size_t)a.nrt * a.nlt;
   const size_t base = 2 * ntp * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
-  size_t lds = base + (size_t)a.nrec * sizeof(float4);
-  a.rec_in_lds = 1;
-  if (lds > d.lds_per_block) {
-    a.rec_in_lds = 0;
-    lds = base;
-  }
-  if (lds > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
-  if (lds > 65536) HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(pose_kernel<MODE>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  if (blocks == 0) {
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pose_kernel<MODE>, wpb * kWave, lds));
-    if (per_cu < 1) per_cu = 1;
-    blocks = per_cu * d.cus;
-    const long long need = ((long long)a.N + wpb - 1) / wpb;
-    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
-  }
-  hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(pose_kernel<MODE>, dim3(blocks), dim3(wpb * kWave), lds, stream, a);
-  HIPCHK(hipGetLastError());
-  return ATJ_OK;
+  if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
+  const size_t with_rec = base + (size_t)a.nrec * sizeof(float4);
+  const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
+  if (with_rec <= d.lds_per_block) return launch_pose_kernel_t<MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  return launch_pose_kernel_t<MODE, false>(a, wpb, base, blocks, d, stream);
 }
 
 }  // namespace

@@ -119,7 +119,7 @@ typedef struct AtjPoseState {
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
 typedef struct AtjLaunch {
-  int32_t waves_per_block;  /* 1..8 (0 = default 4) */
+  int32_t waves_per_block;  /* 1..16 (0 = default 4) */
   int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
 } AtjLaunch;
 

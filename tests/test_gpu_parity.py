@@ -283,7 +283,7 @@ def test_edge_cases(s1):
     E, G = s1.system.energrad(far, s1.ligs[0])
     assert torch.count_nonzero(E) == 0 and torch.count_nonzero(G) == 0
     with pytest.raises(_lib.AttractError):
-        bad = _lib.AtjLaunch(9, 0)
+        bad = _lib.AtjLaunch(17, 0)
         old = s1.system.launch
         s1.system.launch = bad
         try:
