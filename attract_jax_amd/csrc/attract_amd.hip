@@ -46,6 +46,8 @@ constexpr int kWave = 64;
 constexpr int kSeg = 8;            // receptor indices per 16-byte list segment
 constexpr int kMaxWavesPerBlock = 16;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
+constexpr int kPairCost = 36;      // approximate instructions per neighbour slot
+constexpr int kItemCost = 40;      // approximate instructions per list segment
 
 enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2 };
 
@@ -247,25 +249,61 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   for (int i = 0; i < 4; ++i) raw_out[i] = raw[i];
 }
 
-// ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and its dsq-derivative,
+// ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and 2 x its dsq-derivative,
 // optionally minus the plateau energy with the force-shifted gradient (:124-147).
-// pa = (rc, ac, rmin2, ivor), pb = (emin, plateau E, 2*plateau dE/ddsq*plateaudis, -).
-// Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).
-__device__ __forceinline__ void pair_term(float dsq, const float4 pa, const float4 pb, bool plateau,
-                                          bool exact, float& E, float& gs) {
-  const float u = __builtin_amdgcn_rcpf(dsq);
-  const float u3 = (u * u) * u;
+// pa = (rc, ac, rmin2, ivor), pb = ((ivor-1) emin, plateau E, 2 plateau dE/ddsq plateaudis, 0).
+// Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).  An all-zero record
+// gives E = gs = 0 for any dsq > 0 (used for padding and pairs beyond the plateau).
+template <bool PLATEAU, bool EXACT>
+__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float4 pb, float& E, float& gs) {
+  float u, rs = 0.f;
+  if (PLATEAU && !EXACT) {
+    rs = __builtin_amdgcn_rsqf(dsq);
+    u = rs * rs;
+  } else {
+    u = __builtin_amdgcn_rcpf(dsq);
+  }
+  const float u2 = u * u;
   const float rep = pa.x * u;
-  const float vlj = (rep - pa.y) * u3;
-  const float dv = -(fmaf(4.f, rep, -3.f * pa.y) * u3) * u;
+  const float t1 = rep - pa.y;
+  const float vlj = t1 * (u2 * u);
+  // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
+  const float dv2 = fmaf(3.f, t1, rep) * (-2.f * (u2 * u2));
   const bool lo = dsq < pa.z;
   const float s = lo ? 1.f : pa.w;
-  E = lo ? fmaf(pa.w - 1.f, pb.x, vlj) : pa.w * vlj;
-  gs = 2.f * (s * dv);
-  if (plateau) {
+  E = fmaf(s, vlj, lo ? pb.x : 0.f);
+  gs = s * dv2;
+  if (PLATEAU) {
     E -= pb.y;
-    if (!exact) gs = fmaf(-pb.z, __builtin_amdgcn_rsqf(dsq), gs);
+    if (!EXACT) gs = fmaf(-pb.z, rs, gs);
   }
+}
+
+__device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const float4 pa, const float4 pb,
+                                             float& E, float& gs) {
+  if (plateau) {
+    if (exact)
+      pair_eval<true, true>(dsq, pa, pb, E, gs);
+    else
+      pair_eval<true, false>(dsq, pa, pb, E, gs);
+  } else {
+    pair_eval<false, false>(dsq, pa, pb, E, gs);
+  }
+}
+
+// Pair-table record of (rc, ac, ivor, emin, rmin2) at plateaudis^2 = P2.
+__device__ __forceinline__ void pair_record(float rc, float ac, float iv, float em, float r2, float P2, float4& pa,
+                                            float4& pb) {
+  const float u = 1.f / P2;
+  const float u3 = (u * u) * u;
+  const float rep = rc * u;
+  const float vlj = (rep - ac) * u3;
+  const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
+  const bool lo = P2 < r2;
+  const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
+  const float pG = lo ? dv : iv * dv;
+  pa = make_float4(rc, ac, r2, iv);
+  pb = make_float4((iv - 1.f) * em, pE, 2.f * pG * sqrtf(P2), 0.f);
 }
 
 // Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs.
@@ -282,8 +320,7 @@ __device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, floa
 struct WaveScratch {
   float4 xp[kWave];   // x', y', z', ligand ff type (int bits)
   float4 xl[kWave];   // x, y, z (local), list start (int bits)
-  int cnt[kWave];     // list length of the bead
-  int segincl[kWave]; // inclusive scan of segments per bead
+  int segincl[kWave]; // inclusive scan of list segments per bead
 };
 
 struct KArgs {
@@ -318,9 +355,114 @@ struct KArgs {
   unsigned long long* stats;
 };
 
+// Far-away receptor bead standing for list padding (0xFFFF entries): every pair with it
+// lies beyond the plateau and uses the all-zero pair record.
+__device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
+
+// Pair loop over the flattened list segments (S entries each) of one chunk of beads.
+template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+__device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
+                                           const float4* __restrict__ s_par, float* s_pgrad, const WaveScratch* ws,
+                                           int total, float wgt, float acc[kAcc], unsigned& npair, int lane) {
+  const GridView& G = a.G;
+  const float P2 = G.p2;
+  const unsigned nrec = (unsigned)a.nrec;
+  const unsigned nlt = (unsigned)a.nlt;
+  const unsigned ntp = (unsigned)(a.nrt * a.nlt);
+  int jb = 0;
+  for (int base = 0; base < total; base += kWave) {
+    const int item = base + lane;
+    const bool act = item < total;
+    if (act)
+      while (ws->segincl[jb] <= item) ++jb;
+    const int sbase = jb ? ws->segincl[jb - 1] : 0;
+    const float4 bx = ws->xp[jb];
+    const float4 bl = ws->xl[jb];
+    const unsigned blt = (unsigned)__float_as_int(bx.w);
+    // inactive lanes read the all-padding segment at the head of the list array
+    const int off = act ? __float_as_int(bl.w) + (item - sbase) * S : 0;
+    unsigned words[S / 2];
+    if (S == 8) {
+      const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);
+      words[0] = ent.x;
+      words[1] = ent.y;
+      words[S / 2 > 2 ? 2 : 0] = ent.z;
+      words[S / 2 > 3 ? 3 : 0] = ent.w;
+    } else {
+      const uint2 ent = *reinterpret_cast<const uint2*>(G.nblist + off);
+      words[0] = ent.x;
+      words[1] = ent.y;
+    }
+    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+    for (int h = 0; h < S / 4; ++h) {
+      float4 rb[4];
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        const unsigned wd = words[2 * h + (cc >> 1)];
+        const unsigned r = (cc & 1) ? (wd >> 16) : (wd & 0xffffu);
+        if (REC_LDS) {
+          rb[cc] = recp[min(r, nrec)];
+        } else {
+          const float4 v = recp[min(r, nrec - 1u)];
+          rb[cc] = r < nrec ? v : sentinel_bead();
+        }
+      }
+      float dd[4][3], dsq[4];
+      float4 pa[4], pb[4];
+      bool ok[4];
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        dd[cc][0] = bx.x - rb[cc].x;
+        dd[cc][1] = bx.y - rb[cc].y;
+        dd[cc][2] = bx.z - rb[cc].z;
+        dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
+        ok[cc] = dsq[cc] < P2;
+        const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * nlt + blt : ntp;
+        pa[cc] = s_par[2 * pi];
+        pb[cc] = s_par[2 * pi + 1];
+      }
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        float E, gs;
+        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], pb[cc], E, gs);
+        e += E;
+        gx = fmaf(gs, dd[cc][0], gx);
+        gy = fmaf(gs, dd[cc][1], gy);
+        gz = fmaf(gs, dd[cc][2], gz);
+        npair += ok[cc] ? 1u : 0u;
+        if (MODE == kAdjoint && ok[cc]) {
+          float d[3];
+          pair_param_grad(dsq[cc], pa[cc], d);
+          if (PLATEAU) {
+            float dp[3];
+            pair_param_grad(P2, pa[cc], dp);
+            d[0] -= dp[0];
+            d[1] -= dp[1];
+            d[2] -= dp[2];
+          }
+          const unsigned pi = (unsigned)__float_as_int(rb[cc].w) * nlt + blt;
+          atomicAdd(&s_pgrad[pi], wgt * d[0]);
+          atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
+          atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
+        }
+      }
+    }
+    acc[0] += e;
+    const float g[3] = {gx, gy, gz};
+    const float xb[3] = {bl.x, bl.y, bl.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+  }
+}
+
 // Per-pose energy and dE/dM for a uniform pose matrix M; acc[] returns the
-// wave-reduced sums on every lane.
-template <int MODE, bool REC_LDS>
+// wave-reduced sums, identical on every lane.
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               float* s_pgrad, WaveScratch* ws, const float4* __restrict__ lig, int nb,
                               const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
@@ -330,10 +472,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
-  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0;
-  const bool exact = (a.flags & ATJ_GRAD_EXACT) != 0;
   const GridView& G = a.G;
-  const float P2 = G.p2;
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
 
@@ -354,7 +493,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       float E, g[3], raw[4];
       int voxel;
       unsigned in_any;
-      bead_grid(G, gt, xp, do_grid, exact, E, g, voxel, in_any, raw);
+      bead_grid(G, gt, xp, do_grid, EXACT, E, g, voxel, in_any, raw);
       acc[0] += E;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -374,95 +513,25 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       }
     }
     if (!do_nb) continue;
-    const int nseg = (cnt + kSeg - 1) / kSeg;
-    const int incl = wave_incl_scan(nseg);
-    const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
-    if (total == 0) continue;
+    // segment size per chunk: 8 (fewer items) or 4 (less padding / fewer idle lanes)
+    const int incl8 = wave_incl_scan((cnt + 7) >> 3);
+    const int incl4 = wave_incl_scan((cnt + 3) >> 2);
+    const int tot8 = __builtin_amdgcn_readlane(incl8, kWave - 1);
+    const int tot4 = __builtin_amdgcn_readlane(incl4, kWave - 1);
+    if (tot8 == 0) continue;
+    const int cost8 = ((tot8 + kWave - 1) / kWave) * (8 * kPairCost + kItemCost);
+    const int cost4 = ((tot4 + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+    const bool use8 = cost8 <= cost4;
     ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
     ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
-    ws->cnt[lane] = cnt;
-    ws->segincl[lane] = incl;
+    ws->segincl[lane] = use8 ? incl8 : incl4;
     wave_sync();
-    int jb = 0;
-    for (int base = 0; base < total; base += kWave) {
-      const int item = base + lane;
-      const bool act = item < total;
-      if (act)
-        while (ws->segincl[jb] <= item) ++jb;
-      const int sbase = jb ? ws->segincl[jb - 1] : 0;
-      const float4 bx = ws->xp[jb];
-      const float4 bl = ws->xl[jb];
-      const int bcnt = ws->cnt[jb];
-      const int bstart = __float_as_int(bl.w);
-      const unsigned blt = (unsigned)__float_as_int(bx.w);
-      const int k0 = act ? (item - sbase) * kSeg : 0;
-      const int n = act ? min(kSeg, bcnt - k0) : 0;
-      const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + (act ? bstart + k0 : 0));
-      const unsigned words[4] = {ent.x, ent.y, ent.z, ent.w};
-      float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-      // branch-free batches of 4 neighbours: independent LDS reads, masked results
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        float4 rb[4];
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          const unsigned wd = words[2 * h + (cc >> 1)];
-          const unsigned r = (cc & 1) ? (wd >> 16) : (wd & 0xffffu);
-          rb[cc] = recp[(4 * h + cc < n) ? r : 0u];
-        }
-        float dsq[4], dd[4][3];
-        bool ok[4];
-        float4 pa[4], pb[4];
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          dd[cc][0] = bx.x - rb[cc].x;
-          dd[cc][1] = bx.y - rb[cc].y;
-          dd[cc][2] = bx.z - rb[cc].z;
-          dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
-          ok[cc] = (4 * h + cc < n) && (dsq[cc] < P2);
-          const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * (unsigned)a.nlt + blt : 0u;
-          pa[cc] = s_par[2 * pi];
-          pb[cc] = s_par[2 * pi + 1];
-        }
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          float E, gs;
-          pair_term(dsq[cc], pa[cc], pb[cc], plateau, exact, E, gs);
-          E = ok[cc] ? E : 0.f;
-          gs = ok[cc] ? gs : 0.f;
-          e += E;
-          gx = fmaf(gs, dd[cc][0], gx);
-          gy = fmaf(gs, dd[cc][1], gy);
-          gz = fmaf(gs, dd[cc][2], gz);
-          st[3] += ok[cc] ? 1u : 0u;
-          if (MODE == kAdjoint && ok[cc]) {
-            float d[3];
-            pair_param_grad(dsq[cc], pa[cc], d);
-            if (plateau) {
-              float dp[3];
-              pair_param_grad(P2, pa[cc], dp);
-              d[0] -= dp[0];
-              d[1] -= dp[1];
-              d[2] -= dp[2];
-            }
-            const unsigned ntp = (unsigned)(a.nrt * a.nlt);
-            const unsigned pi = (unsigned)__float_as_int(rb[cc].w) * (unsigned)a.nlt + blt;
-            atomicAdd(&s_pgrad[pi], wgt * d[0]);
-            atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
-            atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
-          }
-        }
-      }
-      acc[0] += e;
-      const float g[3] = {gx, gy, gz};
-      const float xb[3] = {bl.x, bl.y, bl.z};
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
-#pragma unroll
-      for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
-    }
+    unsigned npair = 0;
+    if (use8)
+      pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot8, wgt, acc, npair, lane);
+    else
+      pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot4, wgt, acc, npair, lane);
+    st[3] += npair;
     wave_sync();
   }
 #pragma unroll
@@ -484,33 +553,31 @@ __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float 
   }
 }
 
-// Stage the receptor beads and the derived pair table into LDS.
+// Stage the receptor beads (+ the padding sentinel) and the pair table (+ the
+// all-zero record) into LDS.
 __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s_pgrad) {
   const int tid = threadIdx.x;
-  if (s_rec)
+  if (s_rec) {
     for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = a.rec[i];
+    if (tid == 0) s_rec[a.nrec] = sentinel_bead();
+  }
   const int ntp = a.nrt * a.nlt;
   const float P2 = a.G.p2;
-  const float P = sqrtf(P2);
   for (int i = tid; i < ntp; i += blockDim.x) {
-    const float rc = a.rc[i], ac = a.ac[i], iv = a.ivor[i], em = a.emin[i], r2 = a.rmin2[i];
-    // plateau energy / dsq-gradient: nonbon and grad(nonbon) at plateaudis**2
-    const float u = 1.f / P2;
-    const float u3 = (u * u) * u;
-    const float rep = rc * u;
-    const float vlj = (rep - ac) * u3;
-    const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
-    const bool lo = P2 < r2;
-    const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
-    const float pG = lo ? dv : iv * dv;
-    s_par[2 * i] = make_float4(rc, ac, r2, iv);
-    s_par[2 * i + 1] = make_float4(em, pE, 2.f * pG * P, 0.f);
+    float4 pa, pb;
+    pair_record(a.rc[i], a.ac[i], a.ivor[i], a.emin[i], a.rmin2[i], P2, pa, pb);
+    s_par[2 * i] = pa;
+    s_par[2 * i + 1] = pb;
+  }
+  if (tid == 0) {
+    s_par[2 * ntp] = make_float4(0.f, 0.f, 0.f, 0.f);
+    s_par[2 * ntp + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   if (s_pgrad)
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
-template <int MODE, bool REC_LDS>
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
   const int nwaves = blockDim.x / kWave;
@@ -518,8 +585,8 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int lane = threadIdx.x % kWave;
   const int ntp = a.nrt * a.nlt;
   float4* s_par = smem;
-  float4* s_rec = s_par + 2 * ntp;
-  const int nrec_lds = REC_LDS ? a.nrec : 0;
+  float4* s_rec = s_par + 2 * (ntp + 1);
+  const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
   stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_pgrad);
@@ -561,7 +628,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
         quat_to_mat(q, R);
         const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
                              R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
-        pose_energrad<MODE, REC_LDS>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
+        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
         elast = acc[0];
         float A[9];
 #pragma unroll
@@ -602,7 +669,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       float M[16];
       load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
       const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      pose_energrad<MODE, REC_LDS>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
+      pose_energrad<MODE, REC_LDS, PLATEAU, EXACT>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
       if (lane == 0) {
         a.energy[p] = acc[0];
         if (a.dEdM) {
@@ -683,20 +750,10 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
       if (dsq < a.G.p2) {
         const int rt = __float_as_int(rb.w);
         const int pi = rt * a.nlt + lt;
-        const float rc = a.rc[pi], ac = a.ac[pi], iv = a.ivor[pi], em = a.emin[pi], r2 = a.rmin2[pi];
-        const float P2 = a.G.p2;
-        const float u = 1.f / P2;
-        const float u3 = (u * u) * u;
-        const float rep = rc * u;
-        const float vlj = (rep - ac) * u3;
-        const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
-        const bool lo = P2 < r2;
-        const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
-        const float pG = lo ? dv : iv * dv;
-        const float4 pa = make_float4(rc, ac, r2, iv);
-        const float4 pb = make_float4(em, pE, 2.f * pG * sqrtf(P2), 0.f);
+        float4 pa, pb;
+        pair_record(a.rc[pi], a.ac[pi], a.ivor[pi], a.emin[pi], a.rmin2[pi], a.G.p2, pa, pb);
         float Ep, gs;
-        pair_term(dsq, pa, pb, plateau, exact, Ep, gs);
+        pair_eval_rt(plateau, exact, dsq, pa, pb, Ep, gs);
         e += Ep;
         gg[0] = fmaf(gs, ddx, gg[0]);
         gg[1] = fmaf(gs, ddy, gg[1]);
@@ -955,9 +1012,9 @@ int device_info(DevInfo& d) {
   return ATJ_OK;
 }
 
-template <int MODE, bool REC_LDS>
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
-  auto kern = pose_kernel<MODE, REC_LDS>;
+  auto kern = pose_kernel<MODE, REC_LDS, PLATEAU, EXACT>;
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
@@ -976,6 +1033,15 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   return ATJ_OK;
 }
 
+template <int MODE, bool REC_LDS>
+int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
+  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0, exact = (a.flags & ATJ_GRAD_EXACT) != 0;
+  if (plateau && exact) return launch_pose_kernel_t<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
+  if (plateau) return launch_pose_kernel_t<MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
+  if (exact) return launch_pose_kernel_t<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
+  return launch_pose_kernel_t<MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
+}
+
 template <int MODE>
 int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (!a.counter) return fail(ATJ_ERR_INVALID, "work counter is NULL");
@@ -986,12 +1052,13 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   const size_t ntp = (size_t)a.nrt * a.nlt;
-  const size_t base = 2 * ntp * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+  const size_t base =
+      2 * (ntp + 1) * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
-  const size_t with_rec = base + (size_t)a.nrec * sizeof(float4);
+  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  if (with_rec <= d.lds_per_block) return launch_pose_kernel_t<MODE, true>(a, wpb, with_rec, blocks, d, stream);
-  return launch_pose_kernel_t<MODE, false>(a, wpb, base, blocks, d, stream);
+  if (with_rec <= d.lds_per_block) return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
 }
 
 }  // namespace

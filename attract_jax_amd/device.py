@@ -122,7 +122,8 @@ class ForceField:
 
 def lists_from_csr(counts, starts0, flat):
     """Device neighbour-list format from CSR lists (counts, 0-based starts, flat indices):
-    returns (nbgrid (V,2) int32, nblist uint16) with 8-aligned starts and 0xFFFF padding."""
+    returns (nbgrid (V,2) int32, nblist uint16) with 8-aligned starts >= 8, 0xFFFF padding
+    and an all-padding first segment."""
     counts = np.asarray(counts, np.int64).reshape(-1)
     counts = np.maximum(counts, 0)
     starts0 = np.asarray(starts0, np.int64).reshape(-1)
@@ -130,9 +131,10 @@ def lists_from_csr(counts, starts0, flat):
     if len(flat) and int(flat.max()) >= 0xFFFF:
         raise ValueError("neighbour lists need receptor indices < 65535")
     padded = (counts + SEG - 1) // SEG * SEG
-    new_starts = np.concatenate(([0], np.cumsum(padded)[:-1]))
-    total = int(padded.sum())
-    nblist = np.full(max(total, SEG), 0xFFFF, np.uint16)
+    # the first segment is all padding (read by idle lanes); lists follow it
+    new_starts = SEG + np.concatenate(([0], np.cumsum(padded)[:-1]))
+    total = SEG + int(padded.sum())
+    nblist = np.full(total, 0xFFFF, np.uint16)
     n = int(counts.sum())
     if n:
         within = np.arange(n) - np.repeat(np.cumsum(counts) - counts, counts)
@@ -293,10 +295,11 @@ class DeviceGrid:
         _lib.check(lib.atj_grid_count_neighbours(ctypes.byref(geom), ctypes.byref(rstruct),
                                                  float(neighbourdis), _ptr(counts), st))
         padded = (counts.to(torch.int64) + SEG - 1) // SEG * SEG
-        ends = torch.cumsum(padded, 0)
+        ends = torch.cumsum(padded, 0) + SEG
         starts = (ends - padded).contiguous()
-        total = int(ends[-1].item()) if V else 0
-        self.nblist = torch.empty(max(total, SEG), dtype=torch.int16, device=d)
+        total = int(ends[-1].item()) if V else SEG
+        self.nblist = torch.empty(total, dtype=torch.int16, device=d)
+        self.nblist[:SEG].fill_(-1)
         self.nbgrid = torch.empty((V, 2), dtype=torch.int32, device=d)
         _lib.check(lib.atj_grid_fill_neighbours(ctypes.byref(self.struct()), ctypes.byref(rstruct),
                                                 float(neighbourdis), _ptr(counts), _ptr(starts),

@@ -61,8 +61,9 @@ typedef struct AtjGrid {
   const float* inner;     /* potentials + gradients, NULL for a neighbour-only grid */
   const float* outer;     /* NULL = no outer grid */
   const int32_t* nbgrid;  /* [dx*dy*dz][2] = (count, start): 0-based start into nblist,
-                             start % 8 == 0 (lists are 16-byte aligned) */
-  const uint16_t* nblist; /* receptor bead indices, padded with 0xFFFF */
+                             start % 8 == 0 and start >= 8 (lists are 16-byte aligned) */
+  const uint16_t* nblist; /* receptor bead indices; each list padded with 0xFFFF to a multiple
+                             of 8 entries; entries 0..7 are 0xFFFF */
 } AtjGrid;
 
 /* ATTRACT pair table, compacted to receptor types x ligand types

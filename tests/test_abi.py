@@ -48,7 +48,8 @@ def test_device_list_format():
     flat = np.arange(21) * 7
     nbgrid, nblist = lists_from_csr(counts, starts, flat)
     assert np.array_equal(nbgrid[:, 0], counts)
-    assert np.all(nbgrid[:, 1] % SEG == 0)
+    assert np.all(nbgrid[:, 1] % SEG == 0) and np.all(nbgrid[:, 1] >= SEG)
+    assert np.all(nblist[:SEG] == 0xFFFF)
     for v in range(len(counts)):
         c, s = nbgrid[v]
         assert np.array_equal(nblist[s:s + c], flat[starts[v]:starts[v] + c])
