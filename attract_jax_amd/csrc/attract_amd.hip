@@ -430,7 +430,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         gx = fmaf(gs, dd[cc][0], gx);
         gy = fmaf(gs, dd[cc][1], gy);
         gz = fmaf(gs, dd[cc][2], gz);
-        npair += ok[cc] ? 1u : 0u;
+        npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
         if (MODE == kAdjoint && ok[cc]) {
           float d[3];
           pair_param_grad(dsq[cc], pa[cc], d);
@@ -526,12 +526,12 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
     ws->segincl[lane] = use8 ? incl8 : incl4;
     wave_sync();
-    unsigned npair = 0;
+    unsigned npair = 0;  // wave-wide count (scalar), credited to lane 0
     if (use8)
       pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot8, wgt, acc, npair, lane);
     else
       pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot4, wgt, acc, npair, lane);
-    st[3] += npair;
+    st[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
 #pragma unroll
@@ -617,6 +617,12 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       float elast = a.elast[p];
       const float lr = lane < 4 ? job.lr_rot : job.lr_trans;
       const float b1 = job.beta1, b2 = job.beta2;
+      // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
+      float c1 = 0.f, c2 = 0.f;
+      for (int i = 0; i < cnt; ++i) {
+        c1 = fmaf(b1, c1, 1.f - b1);
+        c2 = fmaf(b2, c2, 1.f - b2);
+      }
       float q[4], t[3];
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = lane_value(prm, i);
@@ -635,15 +641,20 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
         for (int i = 0; i < 9; ++i) A[i] = acc[1 + i];
         float gq[4];
         quat_grad(q, A, gq);
-        const float g = lane == 0 ? gq[0] : lane == 1 ? gq[1] : lane == 2 ? gq[2] : lane == 3 ? gq[3]
-                      : lane == 4 ? acc[10] : lane == 5 ? acc[11] : acc[12];
+        float g = acc[12];
+        g = lane == 5 ? acc[11] : g;
+        g = lane == 4 ? acc[10] : g;
+        g = lane == 3 ? gq[3] : g;
+        g = lane == 2 ? gq[2] : g;
+        g = lane == 1 ? gq[1] : g;
+        g = lane == 0 ? gq[0] : g;
         ++cnt;
         // optax.adam: scale_by_adam (bias-corrected, eps outside the sqrt) then -lr
-        const float bc1 = 1.f - powf(b1, (float)cnt);
-        const float bc2 = 1.f - powf(b2, (float)cnt);
+        c1 = uni(fmaf(b1, c1, 1.f - b1));
+        c2 = uni(fmaf(b2, c2, 1.f - b2));
         m1 = (1.f - b1) * g + b1 * m1;
         m2 = (1.f - b2) * (g * g) + b2 * m2;
-        const float upd = (m1 / bc1) / (sqrtf(m2 / bc2) + job.eps);
+        const float upd = (m1 / c1) / (sqrtf(m2 / c2) + job.eps);
         prm = prm + (-lr) * upd;
         // Rotation.from_quat(as_quat()): renormalise the quaternion
         float qn[4];
@@ -654,7 +665,10 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
         for (int i = 0; i < 4; ++i) q[i] = uni(qn[i] / nrm);
 #pragma unroll
         for (int i = 0; i < 3; ++i) t[i] = lane_value(prm, 4 + i);
-        prm = lane == 0 ? q[0] : lane == 1 ? q[1] : lane == 2 ? q[2] : lane == 3 ? q[3] : prm;
+        prm = lane == 3 ? q[3] : prm;
+        prm = lane == 2 ? q[2] : prm;
+        prm = lane == 1 ? q[1] : prm;
+        prm = lane == 0 ? q[0] : prm;
       }
       if (lane < 7) {
         a.pose7[7 * p + lane] = prm;
@@ -783,8 +797,12 @@ __global__ void adam_step_kernel(const float* dEdM, int N, float* pose7, float* 
   gr[5] = G[13];
   gr[6] = G[14];
   const int cnt = count[p] + 1;
-  const float bc1 = 1.f - powf(b1, (float)cnt);
-  const float bc2 = 1.f - powf(b2, (float)cnt);
+  // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
+  float bc1 = 0.f, bc2 = 0.f;
+  for (int i = 0; i < cnt; ++i) {
+    bc1 = fmaf(b1, bc1, 1.f - b1);
+    bc2 = fmaf(b2, bc2, 1.f - b2);
+  }
   float prm[7] = {q[0], q[1], q[2], q[3], P[4], P[5], P[6]};
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
