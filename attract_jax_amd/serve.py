@@ -1,0 +1,396 @@
+"""Serving: a continuous-batching request loop for docking jobs on one GPU.
+
+A job = (receptor, ligand, starting poses, ADAM steps).  Jobs arrive at any time;
+the scheduler
+  * admits them in arrival order while their HBM footprint (pose state, plus the
+    receptor grid if it is not resident) fits the budget, evicting idle resident
+    grids least-recently-used first;
+  * keeps receptor grids resident on the device and gives every resident receptor a
+    pool of pose slots; jobs of the same receptor share the pool, so one persistent
+    launch (atj_minimize) advances all of them together;
+  * runs each receptor group in chunks of `chunk_steps` ADAM steps on its own HIP
+    stream, so groups overlap on the device and newly admitted jobs join the next
+    chunk of their group (no waiting for the running jobs to finish);
+  * retires a job when its steps are done: its results are copied to the host and
+    its slots and memory are freed.
+The policy (this file's Scheduler and SlotPool) is independent of the device: the
+GPU backend drives the HIP kernels, the fake backend in the tests drives a clock.
+"""
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+POSE_STATE_BYTES = (7 + 7 + 7 + 1 + 1) * 4  # pose, m, v, count, energy per pose slot
+PLAN_BYTES = 8                              # pose_job + order entries per slot
+
+
+@dataclass
+class JobSpec:
+    """A docking request.  receptor_key names the receptor (grid); the backend knows how to
+    build / look up its grid.  quat (n,4) / trans (n,3) are the starting poses."""
+    receptor_key: object
+    ligand_coor: np.ndarray
+    ligand_types: np.ndarray
+    quat: np.ndarray
+    trans: np.ndarray
+    nsteps: int
+    lr_rot: float = 1e-3
+    lr_trans: float = 1e-3
+    arrival: float = 0.0
+
+
+@dataclass
+class Job:
+    id: int
+    spec: JobSpec
+    npose: int
+    slot: int = -1
+    steps_done: int = 0
+    state: str = "pending"     # pending -> admitted -> done | failed
+    t_admit: float = None
+    t_done: float = None
+    result: dict = None
+    error: str = None
+    handle: object = None      # backend data (device ligand, ...)
+
+    @property
+    def remaining(self):
+        return self.spec.nsteps - self.steps_done
+
+
+class SlotPool:
+    """First-fit allocator of contiguous pose-slot ranges in [0, capacity)."""
+
+    def __init__(self, capacity):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)]  # sorted (start, length)
+
+    def alloc(self, n):
+        for i, (s, l) in enumerate(self.free):
+            if l >= n:
+                if l == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + n, l - n)
+                return s
+        return -1
+
+    def release(self, start, n):
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, l in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + l)
+            else:
+                merged.append((s, l))
+        self.free = merged
+
+    def grow(self, new_capacity):
+        if new_capacity <= self.capacity:
+            return
+        self.release(self.capacity, new_capacity - self.capacity)
+        self.capacity = int(new_capacity)
+
+    @property
+    def used(self):
+        return self.capacity - sum(l for _, l in self.free)
+
+
+@dataclass
+class Group:
+    """A resident receptor and the jobs currently running against it."""
+    key: object
+    grid_bytes: int
+    pool: SlotPool
+    jobs: list = field(default_factory=list)
+    inflight: object = None      # backend token of the running chunk
+    chunk_jobs: list = field(default_factory=list)
+    chunk_steps: int = 0
+    last_used: float = 0.0
+    handle: object = None        # backend data (device grid, slot arrays, stream)
+    dirty: bool = True           # job membership changed since the last plan
+
+
+class Scheduler:
+    def __init__(self, backend, hbm_budget, chunk_steps=25, initial_slots=1 << 16, clock=time.perf_counter):
+        self.backend = backend
+        self.budget = int(hbm_budget)
+        self.chunk_steps = int(chunk_steps)
+        self.initial_slots = int(initial_slots)
+        self.clock = clock
+        self.pending = deque()
+        self.groups = OrderedDict()   # LRU order: oldest first
+        self.jobs = []
+        self.done = []
+        self.used = 0
+
+    # ------------------------------------------------------------------ bookkeeping
+    def submit(self, spec, now=None):
+        job = Job(len(self.jobs), spec, len(spec.quat))
+        job.spec.arrival = self.clock() if now is None else now
+        self.jobs.append(job)
+        self.pending.append(job)
+        return job
+
+    def _group_bytes(self, capacity):
+        return capacity * (POSE_STATE_BYTES + PLAN_BYTES)
+
+    def _evict_for(self, need, keep=None):
+        """Free idle resident grids (LRU first) until `need` more bytes fit."""
+        for key in list(self.groups):
+            if self.used + need <= self.budget:
+                return True
+            g = self.groups[key]
+            if g is keep or g.jobs or g.inflight is not None:
+                continue
+            self.backend.drop_group(g)
+            self.used -= g.grid_bytes + self._group_bytes(g.pool.capacity)
+            del self.groups[key]
+        return self.used + need <= self.budget
+
+    def _admit(self, now):
+        while self.pending:
+            job = self.pending[0]
+            key = job.spec.receptor_key
+            g = self.groups.get(key)
+            if g is None:
+                cap = max(self.initial_slots, job.npose)
+                need = self.backend.grid_bytes(key) + self._group_bytes(cap)
+                if need > self.budget:
+                    self.pending.popleft()
+                    job.state, job.error = "failed", "job does not fit the HBM budget"
+                    self.done.append(job)
+                    continue
+                if not self._evict_for(need):
+                    return  # wait until running jobs free memory
+                g = Group(key, self.backend.grid_bytes(key), SlotPool(cap))
+                self.backend.make_group(g)
+                self.used += need
+                self.groups[key] = g
+                start = g.pool.alloc(job.npose)
+            else:
+                start = g.pool.alloc(job.npose)
+                if start < 0:
+                    extra = max(job.npose, g.pool.capacity)
+                    need = self._group_bytes(extra)
+                    if g.inflight is not None or not self._evict_for(need, keep=g):
+                        return  # slot arrays are resized only between chunks
+                    self.backend.grow_group(g, g.pool.capacity + extra)
+                    g.pool.grow(g.pool.capacity + extra)
+                    self.used += need
+                    start = g.pool.alloc(job.npose)
+            self.pending.popleft()
+            job.slot = start
+            job.state = "admitted"
+            job.t_admit = now
+            self.backend.load_job(g, job)
+            g.jobs.append(job)
+            g.dirty = True
+            self.groups.move_to_end(g.key)
+
+    def _launch(self, now):
+        for g in list(self.groups.values()):
+            if g.inflight is not None or not g.jobs:
+                continue
+            runnable = [j for j in g.jobs if j.remaining > 0]
+            if not runnable:
+                continue
+            steps = min(self.chunk_steps, max(j.remaining for j in runnable))
+            g.chunk_jobs = list(g.jobs)
+            g.chunk_steps = steps
+            g.inflight = self.backend.launch(g, g.jobs, steps, g.dirty)
+            g.dirty = False
+            g.last_used = now
+            self.groups.move_to_end(g.key)
+
+    def _reap(self, now):
+        for g in list(self.groups.values()):
+            if g.inflight is None or not self.backend.finished(g.inflight):
+                continue
+            g.inflight = None
+            finished = []
+            for j in g.chunk_jobs:
+                j.steps_done += min(g.chunk_steps, j.remaining)
+                if j.remaining <= 0:
+                    finished.append(j)
+            for j in finished:
+                j.result = self.backend.fetch_job(g, j)
+                j.t_done = self.clock()
+                j.state = "done"
+                g.jobs.remove(j)
+                g.pool.release(j.slot, j.npose)
+                self.backend.release_job(g, j)
+                self.done.append(j)
+                g.dirty = True
+
+    def tick(self, now=None):
+        now = self.clock() if now is None else now
+        self._reap(now)
+        self._admit(now)
+        self._launch(now)
+
+    def idle(self):
+        return not self.pending and all(not g.jobs and g.inflight is None for g in self.groups.values())
+
+    def latencies(self):
+        return np.array([j.t_done - j.spec.arrival for j in self.done if j.state == "done"])
+
+
+# ---------------------------------------------------------------------------- GPU backend
+
+class GpuBackend:
+    """Drives the HIP kernels: resident grids, per-receptor slot arrays, one stream per group."""
+
+    def __init__(self, device, receptors, layout=None, waves_per_block=16, nstreams=4):
+        import torch
+        from . import _lib
+        self.torch = torch
+        self.device = torch.device(device)
+        self.receptors = receptors    # key -> callable returning (ScoringSystem) when built
+        self.layout = layout if layout is not None else _lib.ATJ_LAYOUT_CORNER
+        self.waves = waves_per_block
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(nstreams)]
+        self._next_stream = 0
+        self._built = {}
+
+    def grid_bytes(self, key):
+        return self.receptors[key]["grid_bytes"]
+
+    def _system(self, key):
+        if key not in self._built:
+            self._built[key] = self.receptors[key]["build"]()
+        return self._built[key]
+
+    def make_group(self, g):
+        from . import _lib
+        from .minimize import PoseState
+        torch = self.torch
+        system = self._system(g.key)
+        system.launch = _lib.AtjLaunch(self.waves, 0)
+        cap = g.pool.capacity
+        z = torch.zeros
+        state = PoseState(z((cap, 7), device=self.device), z((cap, 7), device=self.device),
+                          z((cap, 7), device=self.device), z(cap, dtype=torch.int32, device=self.device),
+                          z(cap, device=self.device))
+        stream = self.streams[self._next_stream % len(self.streams)]
+        self._next_stream += 1
+        stream.wait_stream(torch.cuda.current_stream(self.device))  # grid built on the current stream
+        counter = torch.zeros(1, dtype=torch.int32, device=self.device)
+        g.handle = dict(system=system, state=state, stream=stream, counter=counter, plan=None)
+
+    def grow_group(self, g, new_capacity):
+        torch = self.torch
+        st = g.handle["state"]
+        cap = st.N
+
+        def grow(t):
+            shape = (new_capacity,) + tuple(t.shape[1:])
+            n = torch.zeros(shape, dtype=t.dtype, device=self.device)
+            n[:cap] = t
+            return n
+        from .minimize import PoseState
+        with torch.cuda.stream(g.handle["stream"]):
+            g.handle["state"] = PoseState(grow(st.pose), grow(st.m), grow(st.v), grow(st.count), grow(st.energy))
+
+    def drop_group(self, g):
+        self._built.pop(g.key, None)
+        g.handle = None
+        self.torch.cuda.empty_cache()
+
+    def load_job(self, g, job):
+        from .device import Ligand
+        torch = self.torch
+        h = g.handle
+        sp = job.spec
+        lig = Ligand.create(sp.ligand_coor, sp.ligand_types - 1, sp.ligand_types - 1, self.device)
+        sl = slice(job.slot, job.slot + job.npose)
+        with torch.cuda.stream(h["stream"]):
+            pose = torch.cat([torch.as_tensor(sp.quat, dtype=torch.float32),
+                              torch.as_tensor(sp.trans, dtype=torch.float32)], 1).pin_memory()
+            h["state"].pose[sl].copy_(pose, non_blocking=True)
+            h["state"].m[sl].zero_()
+            h["state"].v[sl].zero_()
+            h["state"].count[sl].zero_()
+        job.handle = dict(ligand=lig, pinned=pose)
+
+    def _plan(self, g, jobs):
+        torch = self.torch
+        system = g.handle["system"]
+        ligs = [j.handle["ligand"] for j in jobs]
+        import ctypes
+        from . import _lib
+        arr = (_lib.AtjJob * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = _lib.AtjJob(ligs[i].struct(), j.slot, j.npose, j.spec.nsteps, j.spec.lr_rot, j.spec.lr_trans,
+                                 0.9, 0.999, 1e-8)
+        jobs_t = torch.from_numpy(np.frombuffer(bytes(arr), np.uint8).copy()).to(self.device)
+        cap = g.handle["state"].N
+        pose_job = np.zeros(cap, np.int32)
+        order = []
+        for i, j in sorted(enumerate(jobs), key=lambda ij: -ij[1].handle["ligand"].nbeads):
+            pose_job[j.slot:j.slot + j.npose] = i
+            order.append(np.arange(j.slot, j.slot + j.npose, dtype=np.int32))
+        order = np.concatenate(order)
+        return dict(jobs=jobs_t, njobs=len(jobs), pose_job=torch.from_numpy(pose_job).to(self.device),
+                    order=torch.from_numpy(order).to(self.device), N=len(order))
+
+    def launch(self, g, jobs, steps, dirty):
+        import ctypes
+        from . import _lib
+        from .device import _ptr
+        torch = self.torch
+        h = g.handle
+        with torch.cuda.stream(h["stream"]):
+            if dirty or h["plan"] is None:
+                h["plan"] = self._plan(g, jobs)
+            p = h["plan"]
+            system = h["system"]
+            lib = _lib.load()
+            _lib.check(lib.atj_minimize(ctypes.byref(system._gs), ctypes.byref(system._fs),
+                                        ctypes.byref(system._rs), _ptr(p["jobs"]), p["njobs"], _ptr(p["pose_job"]),
+                                        _ptr(p["order"]), p["N"], ctypes.byref(h["state"].struct()), int(steps),
+                                        system.flags, ctypes.byref(system.launch), _ptr(h["counter"]),
+                                        ctypes.c_void_p(0), ctypes.c_void_p(h["stream"].cuda_stream)))
+            ev = torch.cuda.Event()
+            ev.record(h["stream"])
+        return ev
+
+    def finished(self, token):
+        return token.query()
+
+    def fetch_job(self, g, job):
+        h = g.handle
+        sl = slice(job.slot, job.slot + job.npose)
+        with self.torch.cuda.stream(h["stream"]):
+            pose = h["state"].pose[sl].cpu()
+            energy = h["state"].energy[sl].cpu()
+        return dict(pose=pose.numpy(), energy=energy.numpy())
+
+    def release_job(self, g, job):
+        job.handle = None
+
+
+def run_stream(scheduler, specs, poll=0.0005, timeout=600.0):
+    """Submit specs at their arrival times (seconds from now) and run until all are done.
+    Returns (latencies (s), makespan (s), pose-steps)."""
+    t0 = scheduler.clock()
+    order = sorted(specs, key=lambda s: s.arrival)
+    k = 0
+    while True:
+        now = scheduler.clock()
+        while k < len(order) and now - t0 >= order[k].arrival:
+            sp = order[k]
+            scheduler.submit(sp, now=t0 + sp.arrival)
+            k += 1
+        scheduler.tick(now)
+        if k == len(order) and scheduler.idle():
+            break
+        if now - t0 > timeout:
+            raise TimeoutError("serving run exceeded its time limit")
+        time.sleep(poll)
+    makespan = scheduler.clock() - t0
+    pose_steps = sum(j.npose * j.spec.nsteps for j in scheduler.done if j.state == "done")
+    return scheduler.latencies(), makespan, pose_steps

@@ -210,3 +210,69 @@ def build_system(w, device=None, layout=None, keep_plain=False, flags=None, laun
                             **w.grid_kw)
     ligs = [Ligand.create(l.coor, l.types - 1, l.types - 1, device) for l in w.ligands]
     return ScoringSystem(grid, ff, rec, flags=flags, launch=launch), ligs
+
+
+def c5_receptors(n=16, seed=5000):
+    """Receptor pool of the serving workload: 500-4000 beads at the C2 bead density
+    (radius 25 A * (n/2000)^(1/3)), own 32-type pair table, inner grid at 0.9 A covering
+    the receptor plus 22 A, outer 64^3 at 1.8 A; plain fp32 grids of 0.2-1.1 GB."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed + 100 * i)
+        nrec = int(rng.integers(500, 4001))
+        radius = 25.0 * (nrec / 2000.0) ** (1.0 / 3.0)
+        coor = random_receptor(nrec, radius, 2.25, rng)
+        types = rng.integers(1, NTYPES + 1, size=nrec)
+        rc, ac, ivor, rmin, eabs = random_pair_table(rng)
+        d = int(np.ceil(2 * (radius + 22.0) / 0.9)) + 1
+        o = -0.9 * (d - 1) / 2
+        grid_kw = dict(origin=(o, o, o), spacing=0.9, dim=(d, d, d), dim2=(64, 64, 64), gridextension=16,
+                       plateaudis=10.0, neighbourdis=10.8)
+        out.append(dict(key=i, nrec=nrec, radius=radius, coor=coor, types=types, rc=rc, ac=ac, ivor=ivor,
+                        grid_kw=grid_kw))
+    return out
+
+
+def c5_jobs(receptors, njobs=200, rate=20.0, seed=5999, pose_scale=1.0):
+    """Poisson arrivals (mean `rate` jobs/s); each job: a receptor of the pool, a ligand of
+    5-60 beads, log-uniform 1e3..2e5 poses (times pose_scale) around the receptor surface,
+    100-300 ADAM steps (lr 1e-3)."""
+    from .serve import JobSpec
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    specs = []
+    for i in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        r = receptors[int(rng.integers(0, len(receptors)))]
+        nb = int(rng.integers(5, 61))
+        coor = random_chain(nb, np.random.default_rng(seed + 1000 + i))
+        types = rng.integers(1, NTYPES + 1, size=nb)
+        npose = max(1, int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))) * pose_scale))
+        q, tr = posemod.random_poses(npose, np.random.default_rng(seed + 5000 + i), r["radius"],
+                                     r["radius"] + 10.0)
+        specs.append(JobSpec(r["key"], coor, types, q, tr, int(rng.integers(100, 301)), arrival=t))
+    return specs
+
+
+def c5_backend_receptors(receptors, device, layout=None, keep_plain=False):
+    """Backend receptor table: key -> {'grid_bytes', 'build'} for serve.GpuBackend."""
+    from . import _lib
+    from .device import DeviceGrid, ForceField, Receptor
+    from .score import ScoringSystem
+    layout = _lib.ATJ_LAYOUT_CORNER if layout is None else layout
+    table = {}
+    for r in receptors:
+        d = r["grid_kw"]["dim"][0]
+        plain = NTYPES * 16 * (d ** 3 + 64 ** 3)
+        corner = NTYPES * 128 * ((d - 1) ** 3 + 63 ** 3)
+        lists = d ** 3 * 8 + d ** 3 * 2 * 48
+
+        def build(r=r):
+            ff = ForceField.create(r["rc"], r["ac"], r["ivor"], device=device)
+            rec = Receptor.create(r["coor"], r["types"] - 1, device)
+            grid = DeviceGrid.build(rec, ff, np.arange(NTYPES), layout=layout, device=device, keep_plain=keep_plain,
+                                    **r["grid_kw"])
+            return ScoringSystem(grid, ff, rec)
+        table[r["key"]] = dict(grid_bytes=(corner if layout == _lib.ATJ_LAYOUT_CORNER else plain) + lists,
+                               build=build)
+    return table
