@@ -87,8 +87,11 @@ class AtjPoseState(ctypes.Structure):
     ]
 
 
+ATJ_LAUNCH_REC_GLOBAL = 0x1
+
+
 class AtjLaunch(ctypes.Structure):
-    _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks", ctypes.c_int32)]
+    _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks", ctypes.c_int32), ("options", ctypes.c_uint32)]
 
 
 # (name, argtypes) of every exported entry point; tests check the exports against this table

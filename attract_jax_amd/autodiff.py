@@ -85,7 +85,7 @@ def param_adjoint(system, ligand, M, ff, weights, flags, blocks=None, jobs=None)
     energy = torch.empty(N, dtype=torch.float32, device=d)
     if jobs is None:
         jobs = system.jobs_tensor([ligand])
-    launch = _lib.AtjLaunch(system.launch.waves_per_block, blocks)
+    launch = _lib.AtjLaunch(system.launch.waves_per_block, blocks, system.launch.options)
     _lib.check(lib.atj_energrad_adjoint(ctypes.byref(system._gs), ctypes.byref(ff.struct()),
                                         ctypes.byref(system._rs), _ptr(jobs), 1, ctypes.c_void_p(0), _ptr(M), N,
                                         _ptr(weights), _ptr(energy), ctypes.c_void_p(0), _ptr(partial), flags,

@@ -1,7 +1,7 @@
 """Launch-shape autotuner of the persistent pose kernels.
 
 Times a short persistent minimization (a few ADAM steps of a pose sample) for each
-candidate waves-per-block and keeps the fastest; results are cached per process by
+candidate (waves per block, receptor in LDS or read through L1) and keeps the fastest; results are cached per process by
 (receptor size, grid dims, ligand-size histogram bucket, candidate set)."""
 import time
 
@@ -10,7 +10,7 @@ import torch
 from . import _lib
 from .minimize import BatchPlan, PoseState, minimize
 
-CANDIDATES = (4, 8, 16)
+CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _lib.ATJ_LAUNCH_REC_GLOBAL), (8, _lib.ATJ_LAUNCH_REC_GLOBAL))
 _cache = {}
 
 
@@ -29,8 +29,8 @@ def autotune_minimize(system, plan, w, device, candidates=CANDIDATES, steps=5, s
     t = torch.as_tensor(w.trans[idx], dtype=torch.float32)
     old = system.launch
     best, best_t = None, float("inf")
-    for wpb in candidates:
-        system.launch = _lib.AtjLaunch(wpb, 0)
+    for wpb, opt in candidates:
+        system.launch = _lib.AtjLaunch(wpb, 0, opt)
         times = []
         for rep in range(3):
             st = PoseState.create(q, t, device)
@@ -41,9 +41,9 @@ def autotune_minimize(system, plan, w, device, candidates=CANDIDATES, steps=5, s
             times.append(time.perf_counter() - t0)
         tt = min(times[1:])
         if log:
-            log(f"[autotune] waves/block {wpb}: {tt * 1e3:.2f} ms for {sub.N} poses x {steps} steps")
+            log(f"[autotune] waves/block {wpb} options {opt}: {tt * 1e3:.2f} ms for {sub.N} poses x {steps} steps")
         if tt < best_t:
-            best, best_t = wpb, tt
+            best, best_t = (wpb, opt), tt
     system.launch = old
     _cache[key] = best
     return best

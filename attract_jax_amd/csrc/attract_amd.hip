@@ -249,13 +249,42 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   for (int i = 0; i < 4; ++i) raw_out[i] = raw[i];
 }
 
+// Launch-uniform plateau constants: P2 = plateaudis^2 and powers of 1/P2 used to
+// evaluate the plateau energy and its force-shift factor of any pair record.
+struct PlateauConst {
+  float P2, uP3, uP4, k1, k2;  // k1 = -8 P uP^5, k2 = 6 P uP^4 (2 P d vlj/d dsq at P2 = rc k1 + ac k2)
+};
+
+__device__ __forceinline__ PlateauConst plateau_const(float P2) {
+  const float uP = 1.f / P2;
+  const float uP2 = uP * uP;
+  const float uP4 = uP2 * uP2;
+  const float P = sqrtf(P2);
+  PlateauConst c;
+  c.P2 = P2;
+  c.uP3 = uP2 * uP;
+  c.uP4 = uP4;
+  c.k1 = -8.f * P * (uP4 * uP);
+  c.k2 = 6.f * P * uP4;
+  return c;
+}
+
+// 16-byte pair record (rc, ac, ivor * rmin2, (ivor - 1) emin): ivor = +-1 rides on the sign
+// of rmin2 (ATTRACT's ipon is +-1, attract-original/convert-attract-par.py:14-15).  An ivor
+// of another magnitude poisons the record (NaN energies) rather than being misread.
+__device__ __forceinline__ float4 pair_record(float rc, float ac, float iv, float em, float r2) {
+  if (fabsf(iv) != 1.f) rc = __int_as_float(0x7fc00000);
+  return make_float4(rc, ac, iv < 0.f ? -r2 : r2, (iv - 1.f) * em);
+}
+
 // ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and 2 x its dsq-derivative,
-// optionally minus the plateau energy with the force-shifted gradient (:124-147).
-// pa = (rc, ac, rmin2, ivor), pb = ((ivor-1) emin, plateau E, 2 plateau dE/ddsq plateaudis, 0).
-// Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).  An all-zero record
-// gives E = gs = 0 for any dsq > 0 (used for padding and pairs beyond the plateau).
+// optionally minus the plateau energy with the force-shifted gradient (:124-147), the
+// plateau terms evaluated from the record at P2.  Returns E and gs such that
+// dE/dx'_lig = gs * (x'_lig - x_rec).  The all-zero record gives E = gs = 0 for any
+// finite dsq > 0 (padding and pairs beyond the plateau).
 template <bool PLATEAU, bool EXACT>
-__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float4 pb, float& E, float& gs) {
+__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const PlateauConst& pc, float& E,
+                                          float& gs) {
   float u, rs = 0.f;
   if (PLATEAU && !EXACT) {
     rs = __builtin_amdgcn_rsqf(dsq);
@@ -269,52 +298,43 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   const float vlj = t1 * (u2 * u);
   // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
   const float dv2 = fmaf(3.f, t1, rep) * (-2.f * (u2 * u2));
-  const bool lo = dsq < pa.z;
-  const float s = lo ? 1.f : pa.w;
-  E = fmaf(s, vlj, lo ? pb.x : 0.f);
+  const float r2 = fabsf(pa.z);
+  const float iv = pa.z < 0.f ? -1.f : 1.f;
+  const bool lo = dsq < r2;
+  const float s = lo ? 1.f : iv;
+  E = fmaf(s, vlj, lo ? pa.w : 0.f);
   gs = s * dv2;
   if (PLATEAU) {
-    E -= pb.y;
-    if (!EXACT) gs = fmaf(-pb.z, rs, gs);
+    const float vljP = fmaf(pa.x, pc.uP4, -(pa.y * pc.uP3));
+    const bool loP = pc.P2 < r2;
+    const float sP = loP ? 1.f : iv;
+    E -= fmaf(sP, vljP, loP ? pa.w : 0.f);
+    if (!EXACT) gs = fmaf(-(sP * fmaf(pa.x, pc.k1, pa.y * pc.k2)), rs, gs);
   }
 }
 
-__device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const float4 pa, const float4 pb,
-                                             float& E, float& gs) {
+__device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const float4 pa,
+                                             const PlateauConst& pc, float& E, float& gs) {
   if (plateau) {
     if (exact)
-      pair_eval<true, true>(dsq, pa, pb, E, gs);
+      pair_eval<true, true>(dsq, pa, pc, E, gs);
     else
-      pair_eval<true, false>(dsq, pa, pb, E, gs);
+      pair_eval<true, false>(dsq, pa, pc, E, gs);
   } else {
-    pair_eval<false, false>(dsq, pa, pb, E, gs);
+    pair_eval<false, false>(dsq, pa, pc, E, gs);
   }
-}
-
-// Pair-table record of (rc, ac, ivor, emin, rmin2) at plateaudis^2 = P2.
-__device__ __forceinline__ void pair_record(float rc, float ac, float iv, float em, float r2, float P2, float4& pa,
-                                            float4& pb) {
-  const float u = 1.f / P2;
-  const float u3 = (u * u) * u;
-  const float rep = rc * u;
-  const float vlj = (rep - ac) * u3;
-  const float dv = -(fmaf(4.f, rep, -3.f * ac) * u3) * u;
-  const bool lo = P2 < r2;
-  const float pE = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
-  const float pG = lo ? dv : iv * dv;
-  pa = make_float4(rc, ac, r2, iv);
-  pb = make_float4((iv - 1.f) * em, pE, 2.f * pG * sqrtf(P2), 0.f);
 }
 
 // Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs.
 __device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, float d[3]) {
   const float u = 1.f / dsq;
   const float u3 = (u * u) * u;
-  const bool lo = dsq < pa.z;
-  const float s = lo ? 1.f : pa.w;
+  const float iv = pa.z < 0.f ? -1.f : 1.f;
+  const bool lo = dsq < fabsf(pa.z);
+  const float s = lo ? 1.f : iv;
   d[0] = s * (u3 * u);
   d[1] = -s * u3;
-  d[2] = lo ? (pa.w - 1.f) : 0.f;
+  d[2] = lo ? (iv - 1.f) : 0.f;
 }
 
 struct WaveScratch {
@@ -366,6 +386,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
                                            int total, float wgt, float acc[kAcc], unsigned& npair, int lane) {
   const GridView& G = a.G;
   const float P2 = G.p2;
+  const PlateauConst pc = plateau_const(P2);
   const unsigned nrec = (unsigned)a.nrec;
   const unsigned nlt = (unsigned)a.nlt;
   const unsigned ntp = (unsigned)(a.nrt * a.nlt);
@@ -409,7 +430,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         }
       }
       float dd[4][3], dsq[4];
-      float4 pa[4], pb[4];
+      float4 pa[4];
       bool ok[4];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
@@ -419,13 +440,12 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
         ok[cc] = dsq[cc] < P2;
         const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * nlt + blt : ntp;
-        pa[cc] = s_par[2 * pi];
-        pb[cc] = s_par[2 * pi + 1];
+        pa[cc] = s_par[pi];
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         float E, gs;
-        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], pb[cc], E, gs);
+        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], pc, E, gs);
         e += E;
         gx = fmaf(gs, dd[cc][0], gx);
         gy = fmaf(gs, dd[cc][1], gy);
@@ -562,17 +582,9 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s
     if (tid == 0) s_rec[a.nrec] = sentinel_bead();
   }
   const int ntp = a.nrt * a.nlt;
-  const float P2 = a.G.p2;
-  for (int i = tid; i < ntp; i += blockDim.x) {
-    float4 pa, pb;
-    pair_record(a.rc[i], a.ac[i], a.ivor[i], a.emin[i], a.rmin2[i], P2, pa, pb);
-    s_par[2 * i] = pa;
-    s_par[2 * i + 1] = pb;
-  }
-  if (tid == 0) {
-    s_par[2 * ntp] = make_float4(0.f, 0.f, 0.f, 0.f);
-    s_par[2 * ntp + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  for (int i = tid; i < ntp; i += blockDim.x)
+    s_par[i] = pair_record(a.rc[i], a.ac[i], a.ivor[i], a.emin[i], a.rmin2[i]);
+  if (tid == 0) s_par[ntp] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (s_pgrad)
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
@@ -585,7 +597,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int lane = threadIdx.x % kWave;
   const int ntp = a.nrt * a.nlt;
   float4* s_par = smem;
-  float4* s_rec = s_par + 2 * (ntp + 1);
+  float4* s_rec = s_par + (ntp + 1);
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
@@ -764,10 +776,9 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
       if (dsq < a.G.p2) {
         const int rt = __float_as_int(rb.w);
         const int pi = rt * a.nlt + lt;
-        float4 pa, pb;
-        pair_record(a.rc[pi], a.ac[pi], a.ivor[pi], a.emin[pi], a.rmin2[pi], a.G.p2, pa, pb);
+        const float4 pa = pair_record(a.rc[pi], a.ac[pi], a.ivor[pi], a.emin[pi], a.rmin2[pi]);
         float Ep, gs;
-        pair_eval_rt(plateau, exact, dsq, pa, pb, Ep, gs);
+        pair_eval_rt(plateau, exact, dsq, pa, plateau_const(a.G.p2), Ep, gs);
         e += Ep;
         gg[0] = fmaf(gs, ddx, gg[0]);
         gg[1] = fmaf(gs, ddy, gg[1]);
@@ -1071,11 +1082,14 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   const size_t ntp = (size_t)a.nrt * a.nlt;
   const size_t base =
-      2 * (ntp + 1) * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+      (ntp + 1) * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  if (with_rec <= d.lds_per_block) return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  const bool rec_global = launch && (launch->options & ATJ_LAUNCH_REC_GLOBAL);
+  if (!rec_global && with_rec <= d.lds_per_block)
+    return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
   return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
 }
 

@@ -106,6 +106,8 @@ class ForceField:
             emin = -27.0 * ac64 ** 4 / (256.0 * rc64 ** 3)
         if rmin2 is None:
             rmin2 = 4.0 * rc64 / (3.0 * ac64)
+        if not np.all(np.abs(np.asarray(ivor)) == 1):
+            raise ValueError("ivor must be +1 or -1 (ATTRACT ipon)")
         d = _dev(device)
         f = lambda a: torch.as_tensor(np.ascontiguousarray(np.asarray(a, np.float64), ), dtype=torch.float32).to(d)
         return cls(f(rc), f(ac), f(ivor), f(emin), f(rmin2))

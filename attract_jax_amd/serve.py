@@ -269,7 +269,7 @@ class GpuBackend:
         from .minimize import PoseState
         torch = self.torch
         system = self._system(g.key)
-        system.launch = _lib.AtjLaunch(self.waves, 0)
+        system.launch = _lib.AtjLaunch(self.waves, 0, 0)
         cap = g.pool.capacity
         z = torch.zeros
         state = PoseState(z((cap, 7), device=self.device), z((cap, 7), device=self.device),

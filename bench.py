@@ -40,9 +40,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0, help="waves per block (0 = autotuned)")
+    ap.add_argument("--launch-options", type=int, default=0, help="ATJ_LAUNCH_* bits when --waves is given")
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the serving run (0 = skip)")
     ap.add_argument("--serve-receptors", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean job arrivals per second")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (poses shard trivially across "
@@ -57,6 +60,12 @@ def main():
     from attract_jax_amd.workloads import build_system, c3
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     w = c3(args.ligands, args.poses_per_ligand, args.adam_steps)
     log(f"[bench] workload generated in {time.time() - t0:.1f}s: {w.name}")
@@ -75,11 +84,11 @@ def main():
         st.count.zero_()
 
     if args.waves:
-        system.launch = _lib.AtjLaunch(args.waves, 0)
+        system.launch = _lib.AtjLaunch(args.waves, 0, args.launch_options)
     else:
-        best = autotune_minimize(system, plan, w, dev, log=log)
-        system.launch = _lib.AtjLaunch(best, 0)
-    log(f"[bench] waves per block: {system.launch.waves_per_block}")
+        wpb, opt = autotune_minimize(system, plan, w, dev, log=log)
+        system.launch = _lib.AtjLaunch(wpb, 0, opt)
+    log(f"[bench] waves per block: {system.launch.waves_per_block}, options {system.launch.options}")
 
     stream = torch.cuda.current_stream(dev)
     for i in range(args.warmup):
@@ -125,7 +134,8 @@ def main():
         "config": {"workload": w.name, "poses": plan.N, "ligands": args.ligands, "adam_steps": w.nsteps,
                    "receptor_beads": len(w.coor_rec), "grid": "inner 96^3 @0.9A + outer 64^3 @1.8A, 32 types",
                    "layout": "corner-packed" if system.grid.layout == _lib.ATJ_LAYOUT_CORNER else "plain",
-                   "waves_per_block": system.launch.waves_per_block},
+                   "waves_per_block": system.launch.waves_per_block,
+                   "receptor_path": "L1" if system.launch.options & _lib.ATJ_LAUNCH_REC_GLOBAL else "LDS"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "pose_kernel<1> (persistent minimizer)", "kernel_ms": k_ms,

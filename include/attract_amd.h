@@ -71,7 +71,7 @@ typedef struct AtjGrid {
 typedef struct AtjForceField {
   const float* rc;
   const float* ac;
-  const float* ivor;
+  const float* ivor;   /* +1 or -1 (ATTRACT ipon); any other value yields NaN energies */
   const float* emin;   /* -27 ac^4 / (256 rc^3) */
   const float* rmin2;  /* 4 rc / (3 ac) */
   int32_t nrt;
@@ -119,9 +119,11 @@ typedef struct AtjPoseState {
 } AtjPoseState;
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
+#define ATJ_LAUNCH_REC_GLOBAL 0x1u  /* read receptor beads through the vector-memory (L1) path, not LDS */
 typedef struct AtjLaunch {
   int32_t waves_per_block;  /* 1..16 (0 = default 4) */
   int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
+  uint32_t options;         /* ATJ_LAUNCH_* */
 } AtjLaunch;
 
 const char* atj_last_error(void);

@@ -283,10 +283,24 @@ def test_edge_cases(s1):
     E, G = s1.system.energrad(far, s1.ligs[0])
     assert torch.count_nonzero(E) == 0 and torch.count_nonzero(G) == 0
     with pytest.raises(_lib.AttractError):
-        bad = _lib.AtjLaunch(17, 0)
+        bad = _lib.AtjLaunch(17, 0, 0)
         old = s1.system.launch
         s1.system.launch = bad
         try:
             s1.system.energrad(s1.mats(slice(0, 4)), s1.ligs[0])
         finally:
             s1.system.launch = old
+
+
+def test_receptor_paths_bit_identical(s1):
+    """Receptor beads from LDS or through the L1 path: same arithmetic, same results."""
+    mat = s1.mats()
+    old = s1.system.launch
+    try:
+        s1.system.launch = _lib.AtjLaunch(8, 0, 0)
+        E1, G1 = s1.system.energrad(mat, s1.ligs[0])
+        s1.system.launch = _lib.AtjLaunch(8, 0, _lib.ATJ_LAUNCH_REC_GLOBAL)
+        E2, G2 = s1.system.energrad(mat, s1.ligs[0])
+    finally:
+        s1.system.launch = old
+    assert torch.equal(E1, E2) and torch.equal(G1, G2)
