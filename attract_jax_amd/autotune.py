@@ -1,8 +1,9 @@
 """Launch-shape autotuner of the persistent pose kernels.
 
 Times a short persistent minimization (a few ADAM steps of a pose sample) for each
-candidate (waves per block, receptor in LDS or read through L1) and keeps the fastest; results are cached per process by
-(receptor size, grid dims, ligand-size histogram bucket, candidate set)."""
+candidate (waves per block, receptor in LDS or read through L1, one pose per wavefront or
+two) and keeps the fastest; results are cached per process by (receptor size, grid dims,
+ligand-size histogram bucket, candidate set)."""
 import time
 
 import torch
@@ -10,7 +11,9 @@ import torch
 from . import _lib
 from .minimize import BatchPlan, PoseState, minimize
 
-CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _lib.ATJ_LAUNCH_REC_GLOBAL), (8, _lib.ATJ_LAUNCH_REC_GLOBAL))
+_G32 = _lib.ATJ_LAUNCH_GROUP32
+_L1 = _lib.ATJ_LAUNCH_REC_GLOBAL
+CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _L1), (8, _L1), (8, _G32), (16, _G32), (16, _G32 | _L1))
 _cache = {}
 
 

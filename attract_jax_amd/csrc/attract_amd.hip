@@ -111,6 +111,116 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
+// A pose is owned by a group of GS lanes (GS = 64: one pose per wave; GS = 32: two).
+// Group-uniform values: scalar registers for GS = 64, per-lane copies otherwise.
+template <int GS>
+__device__ __forceinline__ float guni(float v) {
+  if (GS == kWave) return uni(v);
+  return v;
+}
+template <int GS>
+__device__ __forceinline__ int guni(int v) {
+  if (GS == kWave) return uni(v);
+  return v;
+}
+
+// Value of group lane `src` of the caller's group.
+template <int GS>
+__device__ __forceinline__ float group_value(float v, int src, int lane) {
+  if (GS == kWave) return lane_value(v, src);
+  const float a0 = lane_value(v, src), a1 = lane_value(v, 32 + src);
+  return lane < 32 ? a0 : a1;
+}
+template <int GS>
+__device__ __forceinline__ int group_value_i(int v, int src, int lane) {
+  if (GS == kWave) return __builtin_amdgcn_readlane(v, src);
+  const int a0 = __builtin_amdgcn_readlane(v, src), a1 = __builtin_amdgcn_readlane(v, 32 + src);
+  return lane < 32 ? a0 : a1;
+}
+
+// Sum over the group in a fixed order (quad, half-row and row mirrors, row broadcasts).
+template <int GS>
+__device__ __forceinline__ float group_sum(float v, int lane) {
+  if (GS == kWave) return wave_sum(v);
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
+  v += dpp<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3: the 32-lane sums
+  return group_value<GS>(v, 31, lane);
+}
+
+// Inclusive prefix sum over the group.
+template <int GS>
+__device__ __forceinline__ int group_incl_scan(int v) {
+  if (GS == kWave) return wave_incl_scan(v);
+  v += dppi<0x111>(v);
+  v += dppi<0x112>(v);
+  v += dppi<0x114>(v);
+  v += dppi<0x118>(v);
+  v += dppi<0x142, 0xA>(v);
+  return v;
+}
+
+// Sums over the group of all kAcc accumulators at once (a transposed butterfly): each
+// halving stage pairs two accumulators, one lane half keeps the pair sum of the first and
+// the other half that of the second, so a pair costs one lane swap and one add.  Stages:
+// lanes l <-> l^32 (GS = 64 only), rows 0 <-> 1 and 2 <-> 3, then in-row mirrors
+// l <-> 15-l and l <-> 7-l, then quad swaps l <-> l^2, l <-> l^1.  Accumulator i ends in the
+// lane whose bits (5, 4, 3, 2) [GS = 64] or (4, 3, 2, 1) [GS = 32] spell out i's bits
+// (0, 1, 2, 3); the order of the additions is fixed, so results are reproducible.
+__device__ __forceinline__ void swap_add(float a, float b, float& out, bool rows32) {
+  const auto r = rows32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                        : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  out = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float mirror_add(float a, float b, bool upper) {
+  return (upper ? b : a) + dpp<CTRL>(upper ? a : b);
+}
+
+template <int GS>
+__device__ __forceinline__ void group_sum_all(float acc[kAcc], int lane) {
+  static_assert(kAcc == 13, "reduction schedule is laid out for 13 accumulators");
+  float s1[7], s2[4], s3[2];
+  if (GS == kWave) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) swap_add(acc[2 * k], 2 * k + 1 < kAcc ? acc[2 * k + 1] : 0.f, s1[k], true);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) swap_add(s1[2 * k], 2 * k + 1 < 7 ? s1[2 * k + 1] : 0.f, s2[k], false);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) s3[k] = mirror_add<0x140>(s2[2 * k], s2[2 * k + 1], lane & 8);  // row_mirror
+    float u = mirror_add<0x141>(s3[0], s3[1], lane & 4);                                        // row_half_mirror
+    u += dpp<0x4E>(u);  // quad_perm [2,3,0,1]
+    u += dpp<0xB1>(u);  // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int i = 0; i < kAcc; ++i)
+      acc[i] = lane_value(u, 32 * (i & 1) + 16 * ((i >> 1) & 1) + 8 * ((i >> 2) & 1) + 4 * (i >> 3));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) swap_add(acc[2 * k], 2 * k + 1 < kAcc ? acc[2 * k + 1] : 0.f, s1[k], false);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s2[k] = mirror_add<0x140>(s1[2 * k], 2 * k + 1 < 7 ? s1[2 * k + 1] : 0.f, lane & 8);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) s3[k] = mirror_add<0x141>(s2[2 * k], s2[2 * k + 1], lane & 4);
+    float u = mirror_add<0x4E>(s3[0], s3[1], lane & 2);  // quad_perm [2,3,0,1]
+    u += dpp<0xB1>(u);                                   // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int i = 0; i < kAcc; ++i)
+      acc[i] = group_value<GS>(u, 16 * (i & 1) + 8 * ((i >> 1) & 1) + 4 * ((i >> 2) & 1) + 2 * (i >> 3), lane);
+  }
+}
+
+// Number of lanes of the caller's group with `pred` set (scalar bit counts).
+template <int GS>
+__device__ __forceinline__ unsigned group_count(bool pred, int lane) {
+  const unsigned long long b = __builtin_amdgcn_ballot_w64(pred);
+  if (GS == kWave) return (unsigned)__builtin_popcountll(b);
+  const unsigned c0 = (unsigned)__builtin_popcount((unsigned)b), c1 = (unsigned)__builtin_popcount((unsigned)(b >> 32));
+  return lane < 32 ? c0 : c1;
+}
+
 // Rotation matrix of a (not necessarily normalised) quaternion (x, y, z, w):
 // the polynomial of scipy / jax.scipy Rotation.as_matrix, used by test4.py:399.
 __device__ __forceinline__ void quat_to_mat(const float q[4], float R[9]) {
@@ -147,6 +257,7 @@ __device__ __forceinline__ void transform(const float M[16], float x, float y, f
 struct GridView {
   float ox, oy, oz, spacing, ge, p2;
   int dx, dy, dz, dx2, dy2, dz2, ntypes, layout;
+  unsigned slab, slab2;  // records (corner) or points (plain) per atom type, inner / outer
   const float* inner;
   const float* outer;
   const int2* nbgrid;
@@ -156,28 +267,29 @@ struct GridView {
 // Trilinear interpolation over one cell (test-grid-energrad.py:183-194):
 // x0 = floor(v), x1 = ceil(v), w0 = x1 - v, w1 = v - x0; corners x -> y -> z.
 // raw = sum_c w_c G_c (4 channels); dEdv = d(sum_c w_c E_c)/dv (exact mode).
-__device__ __forceinline__ void trilinear(const float* g, int layout, int t, int dx, int dy, int dz,
-                                          float vx, float vy, float vz, bool exact, float raw[4],
-                                          float dEdv[3]) {
+// Cell and point indices within one type's slab are 32-bit; one 64-bit multiply-add
+// per bead places the slab.  Requires 0 <= v < dim - 1 on every axis.
+template <int LAYOUT>
+__device__ __forceinline__ void trilinear(const float* g, unsigned slab, int t, int dx, int dy, int dz, float vx,
+                                          float vy, float vz, bool exact, float raw[4], float dEdv[3]) {
   const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
   const float cx = ceilf(vx), cy = ceilf(vy), cz = ceilf(vz);
-  const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+  const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, z0 = (unsigned)fz;
   const float wx[2] = {cx - vx, vx - fx};
   const float wy[2] = {cy - vy, vy - fy};
   const float wz[2] = {cz - vz, vz - fz};
   float4 c[8];
-  if (layout == ATJ_LAYOUT_CORNER) {
-    const size_t rec = (((size_t)t * (dx - 1) + x0) * (dy - 1) + y0) * (size_t)(dz - 1) + z0;
-    const float* p = g + rec * 32;
+  if (LAYOUT == ATJ_LAYOUT_CORNER) {
+    const unsigned cell = (x0 * (unsigned)(dy - 1) + y0) * (unsigned)(dz - 1) + z0;
+    const float4* p = reinterpret_cast<const float4*>(g) + ((size_t)(unsigned)t * slab + cell) * 8;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) c[i] = *reinterpret_cast<const float4*>(p + 4 * i);
+    for (int i = 0; i < 8; ++i) c[i] = p[i];
   } else {
+    const unsigned pt = (x0 * (unsigned)dy + y0) * (unsigned)dz + z0;
+    const float4* p = reinterpret_cast<const float4*>(g) + ((size_t)(unsigned)t * slab + pt);
+    const unsigned sy = (unsigned)dz, sx = (unsigned)(dy * dz);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int ix = x0 + (i >> 2), iy = y0 + ((i >> 1) & 1), iz = z0 + (i & 1);
-      const size_t idx = (((size_t)t * dx + ix) * dy + iy) * (size_t)dz + iz;
-      c[i] = *reinterpret_cast<const float4*>(g + idx * 4);
-    }
+    for (int i = 0; i < 8; ++i) c[i] = p[(i >> 2) * sx + ((i >> 1) & 1) * sy + (i & 1)];
   }
   raw[0] = raw[1] = raw[2] = raw[3] = 0.f;
   dEdv[0] = dEdv[1] = dEdv[2] = 0.f;
@@ -200,6 +312,7 @@ __device__ __forceinline__ void trilinear(const float* g, int layout, int t, int
 
 // Grid term of one bead at x'.  Returns E and dE/dx'; inner voxel index for the
 // neighbour lookup (-1 outside the inner grid, test-grid-energrad.py:216-218,246-253).
+template <int LAYOUT>
 __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const float xp[3], bool do_grid,
                                           bool exact, float& E, float g[3], int& voxel,
                                           unsigned& in_any, float raw_out[4]) {
@@ -221,16 +334,16 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   float raw[4], dEdv[3];
   float inv = 0.f;
   if (inside) {
-    trilinear(G.inner, G.layout, gtype, G.dx, G.dy, G.dz, vx, vy, vz, exact, raw, dEdv);
-    inv = 1.f / G.spacing;
+    trilinear<LAYOUT>(G.inner, G.slab, gtype, G.dx, G.dy, G.dz, vx, vy, vz, exact, raw, dEdv);
+    if (exact) inv = 1.f / G.spacing;
     in_any = 1;
   } else if (G.outer != nullptr) {
     const float ox = (vx + G.ge) / 2.f, oy = (vy + G.ge) / 2.f, oz = (vz + G.ge) / 2.f;
     const bool in_outer = (ox >= 0.f) && (ox < (float)(G.dx2 - 1)) && (oy >= 0.f) &&
                           (oy < (float)(G.dy2 - 1)) && (oz >= 0.f) && (oz < (float)(G.dz2 - 1));
     if (!in_outer) return;
-    trilinear(G.outer, G.layout, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
-    inv = 0.5f / G.spacing;
+    trilinear<LAYOUT>(G.outer, G.slab2, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
+    if (exact) inv = 0.5f / G.spacing;
     in_any = 1;
   } else {
     return;
@@ -337,10 +450,11 @@ __device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, floa
   d[2] = lo ? (iv - 1.f) : 0.f;
 }
 
-struct WaveScratch {
-  float4 xp[kWave];   // x', y', z', ligand ff type (int bits)
-  float4 xl[kWave];   // x, y, z (local), list start (int bits)
-  int segincl[kWave]; // inclusive scan of list segments per bead
+template <int GS>
+struct GroupScratch {
+  float4 xp[GS];   // x', y', z', ligand ff type (int bits)
+  float4 xl[GS];   // x, y, z (local), list start (int bits)
+  int segincl[GS]; // inclusive scan of list segments per bead
 };
 
 struct KArgs {
@@ -380,10 +494,12 @@ struct KArgs {
 __device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
 
 // Pair loop over the flattened list segments (S entries each) of one chunk of beads.
-template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int GS, int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
-                                           const float4* __restrict__ s_par, float* s_pgrad, const WaveScratch* ws,
-                                           int total, float wgt, float acc[kAcc], unsigned& npair, int lane) {
+                                           const float4* __restrict__ s_par, float* s_pgrad,
+                                           const GroupScratch<GS>* ws, int total, float wgt, float acc[kAcc],
+                                           unsigned& npair, int lane) {
+  const int gl = lane & (GS - 1);
   const GridView& G = a.G;
   const float P2 = G.p2;
   const PlateauConst pc = plateau_const(P2);
@@ -391,8 +507,8 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   const unsigned nlt = (unsigned)a.nlt;
   const unsigned ntp = (unsigned)(a.nrt * a.nlt);
   int jb = 0;
-  for (int base = 0; base < total; base += kWave) {
-    const int item = base + lane;
+  for (int base = 0; base < total; base += GS) {
+    const int item = base + gl;
     const bool act = item < total;
     if (act)
       while (ws->segincl[jb] <= item) ++jb;
@@ -450,7 +566,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         gx = fmaf(gs, dd[cc][0], gx);
         gy = fmaf(gs, dd[cc][1], gy);
         gz = fmaf(gs, dd[cc][2], gz);
-        npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
+        npair += group_count<GS>(ok[cc], lane);
         if (MODE == kAdjoint && ok[cc]) {
           float d[3];
           pair_param_grad(dsq[cc], pa[cc], d);
@@ -482,22 +598,23 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
 
 // Per-pose energy and dE/dM for a uniform pose matrix M; acc[] returns the
 // wave-reduced sums, identical on every lane.
-template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              float* s_pgrad, WaveScratch* ws, const float4* __restrict__ lig, int nb,
+                              float* s_pgrad, GroupScratch<GS>* ws, const float4* __restrict__ lig, int nb,
                               const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
+  const int gl = lane & (GS - 1);
   float M[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
+  for (int i = 0; i < 16; ++i) M[i] = guni<GS>(Min[i]);
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
   const GridView& G = a.G;
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
 
-  for (int c0 = 0; c0 < nb; c0 += kWave) {
-    const int j = c0 + lane;
+  for (int c0 = 0; c0 < nb; c0 += GS) {
+    const int j = c0 + gl;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
     int lt = 0;
@@ -513,7 +630,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       float E, g[3], raw[4];
       int voxel;
       unsigned in_any;
-      bead_grid(G, gt, xp, do_grid, EXACT, E, g, voxel, in_any, raw);
+      bead_grid<LAYOUT>(G, gt, xp, do_grid, EXACT, E, g, voxel, in_any, raw);
       acc[0] += E;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -534,28 +651,27 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     if (!do_nb) continue;
     // segment size per chunk: 8 (fewer items) or 4 (less padding / fewer idle lanes)
-    const int incl8 = wave_incl_scan((cnt + 7) >> 3);
-    const int incl4 = wave_incl_scan((cnt + 3) >> 2);
-    const int tot8 = __builtin_amdgcn_readlane(incl8, kWave - 1);
-    const int tot4 = __builtin_amdgcn_readlane(incl4, kWave - 1);
+    const int incl8 = group_incl_scan<GS>((cnt + 7) >> 3);
+    const int incl4 = group_incl_scan<GS>((cnt + 3) >> 2);
+    const int tot8 = group_value_i<GS>(incl8, GS - 1, lane);
+    const int tot4 = group_value_i<GS>(incl4, GS - 1, lane);
     if (tot8 == 0) continue;
-    const int cost8 = ((tot8 + kWave - 1) / kWave) * (8 * kPairCost + kItemCost);
-    const int cost4 = ((tot4 + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+    const int cost8 = ((tot8 + GS - 1) / GS) * (8 * kPairCost + kItemCost);
+    const int cost4 = ((tot4 + GS - 1) / GS) * (4 * kPairCost + kItemCost);
     const bool use8 = cost8 <= cost4;
-    ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-    ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
-    ws->segincl[lane] = use8 ? incl8 : incl4;
+    ws->xp[gl] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
+    ws->xl[gl] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
+    ws->segincl[gl] = use8 ? incl8 : incl4;
     wave_sync();
-    unsigned npair = 0;  // wave-wide count (scalar), credited to lane 0
+    unsigned npair = 0;  // group-wide count, credited to the group's lane 0
     if (use8)
-      pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot8, wgt, acc, npair, lane);
+      pair_items<GS, 8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot8, wgt, acc, npair, lane);
     else
-      pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot4, wgt, acc, npair, lane);
-    st[3] += lane == 0 ? npair : 0u;
+      pair_items<GS, 4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot4, wgt, acc, npair, lane);
+    st[3] += gl == 0 ? npair : 0u;
     wave_sync();
   }
-#pragma unroll
-  for (int i = 0; i < kAcc; ++i) acc[i] = wave_sum(acc[i]);
+  group_sum_all<GS>(acc, lane);
 }
 
 __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float M[16]) {
@@ -589,28 +705,30 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
-template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
+  constexpr int kGroups = kWave / GS;  // poses in flight per wave
   const int nwaves = blockDim.x / kWave;
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
+  const int gl = lane & (GS - 1);
   const int ntp = a.nrt * a.nlt;
   float4* s_par = smem;
   float4* s_rec = s_par + (ntp + 1);
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
-  WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
-  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
+  GroupScratch<GS>* s_ws = reinterpret_cast<GroupScratch<GS>*>(s_rec + nrec_lds);
+  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves * kGroups) : nullptr;
   stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_pgrad);
   __syncthreads();
-  WaveScratch* ws = s_ws + wave;
+  GroupScratch<GS>* ws = s_ws + wave * kGroups + lane / GS;
   const bool qt = (a.flags & ATJ_POSE_QT) != 0;
   unsigned st[4] = {0, 0, 0, 0};
 
   for (;;) {
     int k = 0;
-    if (lane == 0) k = atomicAdd(a.counter, 1);
-    k = __builtin_amdgcn_readfirstlane(k);
+    if (gl == 0) k = atomicAdd(a.counter, 1);
+    k = GS == kWave ? __builtin_amdgcn_readfirstlane(k) : group_value_i<GS>(k, 0, lane);
     if (k >= a.N) break;
     const int p = a.pose_order ? a.pose_order[k] : k;
     const int jid = a.pose_job ? a.pose_job[p] : 0;
@@ -619,15 +737,15 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     const int nb = job.ligand.nbeads;
     float acc[kAcc];
     if (MODE == kMinimize) {
-      // Lane i < 7 owns parameter i of (qx, qy, qz, qw, tx, ty, tz) and its ADAM moments;
-      // the pose itself is uniform (scalar registers) on every lane.
-      const int pi = lane < 7 ? lane : 6;
+      // Group lane i < 7 owns parameter i of (qx, qy, qz, qw, tx, ty, tz) and its ADAM
+      // moments; the pose itself is uniform over the group.
+      const int pi = gl < 7 ? gl : 6;
       float prm = a.pose7[7 * p + pi];
       float m1 = a.mom1[7 * p + pi];
       float m2 = a.mom2[7 * p + pi];
-      int cnt = uni(a.count[p]);
+      int cnt = guni<GS>(a.count[p]);
       float elast = a.elast[p];
-      const float lr = lane < 4 ? job.lr_rot : job.lr_trans;
+      const float lr = gl < 4 ? job.lr_rot : job.lr_trans;
       const float b1 = job.beta1, b2 = job.beta2;
       // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
       float c1 = 0.f, c2 = 0.f;
@@ -637,16 +755,17 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       }
       float q[4], t[3];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) q[i] = lane_value(prm, i);
+      for (int i = 0; i < 4; ++i) q[i] = group_value<GS>(prm, i, lane);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) t[i] = lane_value(prm, 4 + i);
+      for (int i = 0; i < 3; ++i) t[i] = group_value<GS>(prm, 4 + i, lane);
       const int nst = min(a.nsteps_launch, job.nsteps - cnt);
       for (int s = 0; s < nst; ++s) {
         float R[9];
         quat_to_mat(q, R);
         const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
                              R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
-        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st, lane);
+        pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st,
+                                                         lane);
         elast = acc[0];
         float A[9];
 #pragma unroll
@@ -654,40 +773,39 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
         float gq[4];
         quat_grad(q, A, gq);
         float g = acc[12];
-        g = lane == 5 ? acc[11] : g;
-        g = lane == 4 ? acc[10] : g;
-        g = lane == 3 ? gq[3] : g;
-        g = lane == 2 ? gq[2] : g;
-        g = lane == 1 ? gq[1] : g;
-        g = lane == 0 ? gq[0] : g;
+        g = gl == 5 ? acc[11] : g;
+        g = gl == 4 ? acc[10] : g;
+        g = gl == 3 ? gq[3] : g;
+        g = gl == 2 ? gq[2] : g;
+        g = gl == 1 ? gq[1] : g;
+        g = gl == 0 ? gq[0] : g;
         ++cnt;
         // optax.adam: scale_by_adam (bias-corrected, eps outside the sqrt) then -lr
-        c1 = uni(fmaf(b1, c1, 1.f - b1));
-        c2 = uni(fmaf(b2, c2, 1.f - b2));
+        c1 = guni<GS>(fmaf(b1, c1, 1.f - b1));
+        c2 = guni<GS>(fmaf(b2, c2, 1.f - b2));
         m1 = (1.f - b1) * g + b1 * m1;
         m2 = (1.f - b2) * (g * g) + b2 * m2;
         const float upd = (m1 / c1) / (sqrtf(m2 / c2) + job.eps);
         prm = prm + (-lr) * upd;
-        // Rotation.from_quat(as_quat()): renormalise the quaternion
+        // Rotation.from_quat(as_quat()): renormalise the quaternion (lanes 0-3 each divide
+        // their own component)
         float qn[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) qn[i] = lane_value(prm, i);
+        for (int i = 0; i < 4; ++i) qn[i] = group_value<GS>(prm, i, lane);
         const float nrm = sqrtf(fmaf(qn[3], qn[3], fmaf(qn[2], qn[2], fmaf(qn[1], qn[1], qn[0] * qn[0]))));
+        const float pn = prm / nrm;
+        prm = gl < 4 ? pn : prm;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = uni(qn[i] / nrm);
+        for (int i = 0; i < 4; ++i) q[i] = group_value<GS>(prm, i, lane);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) t[i] = lane_value(prm, 4 + i);
-        prm = lane == 3 ? q[3] : prm;
-        prm = lane == 2 ? q[2] : prm;
-        prm = lane == 1 ? q[1] : prm;
-        prm = lane == 0 ? q[0] : prm;
+        for (int i = 0; i < 3; ++i) t[i] = group_value<GS>(prm, 4 + i, lane);
       }
-      if (lane < 7) {
-        a.pose7[7 * p + lane] = prm;
-        a.mom1[7 * p + lane] = m1;
-        a.mom2[7 * p + lane] = m2;
+      if (gl < 7) {
+        a.pose7[7 * p + gl] = prm;
+        a.mom1[7 * p + gl] = m1;
+        a.mom2[7 * p + gl] = m2;
       }
-      if (lane == 0) {
+      if (gl == 0) {
         a.count[p] = cnt;
         a.elast[p] = elast;
       }
@@ -695,8 +813,8 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       float M[16];
       load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
       const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      pose_energrad<MODE, REC_LDS, PLATEAU, EXACT>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
-      if (lane == 0) {
+      pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
+      if (gl == 0) {
         a.energy[p] = acc[0];
         if (a.dEdM) {
           float* o = a.dEdM + (size_t)p * 16;
@@ -759,7 +877,10 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
   float E, g[3], raw[4];
   int voxel;
   unsigned in_any;
-  bead_grid(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
+  if (a.G.layout == ATJ_LAYOUT_CORNER)
+    bead_grid<ATJ_LAYOUT_CORNER>(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
+  else
+    bead_grid<ATJ_LAYOUT_PLAIN>(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
   if (grid4) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) grid4[idx * 4 + i] = raw[i];
@@ -985,6 +1106,19 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.dz2 = g->dim2[2];
   G.ntypes = g->ntypes;
   G.layout = g->layout;
+  // per-type slab sizes; voxel and cell indices within a slab are 32-bit
+  const bool corner = g->layout == ATJ_LAYOUT_CORNER;
+  const unsigned long long pts = (unsigned long long)g->dim[0] * g->dim[1] * g->dim[2];
+  const unsigned long long cells = (unsigned long long)(g->dim[0] - 1) * (g->dim[1] - 1) * (g->dim[2] - 1);
+  if (pts >= (1ull << 31)) return fail(ATJ_ERR_UNSUPPORTED, "inner grid has 2^31 or more points");
+  G.slab = (unsigned)(corner ? cells : pts);
+  G.slab2 = 0;
+  if (g->outer) {
+    const unsigned long long pts2 = (unsigned long long)g->dim2[0] * g->dim2[1] * g->dim2[2];
+    const unsigned long long cells2 = (unsigned long long)(g->dim2[0] - 1) * (g->dim2[1] - 1) * (g->dim2[2] - 1);
+    if (pts2 >= (1ull << 31)) return fail(ATJ_ERR_UNSUPPORTED, "outer grid has 2^31 or more points");
+    G.slab2 = (unsigned)(corner ? cells2 : pts2);
+  }
   G.inner = g->inner;
   G.outer = g->outer;
   G.nbgrid = reinterpret_cast<const int2*>(g->nbgrid);
@@ -1041,9 +1175,9 @@ int device_info(DevInfo& d) {
   return ATJ_OK;
 }
 
-template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
-  auto kern = pose_kernel<MODE, REC_LDS, PLATEAU, EXACT>;
+  auto kern = pose_kernel<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>;
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
@@ -1052,7 +1186,8 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
     if (per_cu < 1) per_cu = 1;
     blocks = per_cu * d.cus;
-    const long long need = ((long long)a.N + wpb - 1) / wpb;
+    const long long per_block = (long long)wpb * (kWave / GS);
+    const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
@@ -1062,13 +1197,30 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   return ATJ_OK;
 }
 
-template <int MODE, bool REC_LDS>
+template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+int launch_layout(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
+  if (a.G.layout == ATJ_LAYOUT_CORNER)
+    return launch_pose_kernel_t<GS, MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_CORNER>(a, wpb, lds, blocks, d, stream);
+  return launch_pose_kernel_t<GS, MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_PLAIN>(a, wpb, lds, blocks, d, stream);
+}
+
+template <int GS, int MODE, bool REC_LDS>
 int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
   const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0, exact = (a.flags & ATJ_GRAD_EXACT) != 0;
-  if (plateau && exact) return launch_pose_kernel_t<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
-  if (plateau) return launch_pose_kernel_t<MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
-  if (exact) return launch_pose_kernel_t<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
-  return launch_pose_kernel_t<MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
+  if (plateau && exact) return launch_layout<GS, MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
+  if (plateau) return launch_layout<GS, MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
+  if (exact) return launch_layout<GS, MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
+  return launch_layout<GS, MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
+}
+
+template <int GS, int MODE>
+int launch_paths(KArgs& a, int wpb, size_t base, bool rec_global, int blocks, const DevInfo& d,
+                 hipStream_t stream) {
+  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
+  if (!rec_global && with_rec <= d.lds_per_block)
+    return launch_flags<GS, MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
+  return launch_flags<GS, MODE, false>(a, wpb, base, blocks, d, stream);
 }
 
 template <int MODE>
@@ -1080,17 +1232,18 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (rc) return rc;
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
+  const uint32_t options = launch ? launch->options : 0u;
+  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_GROUP32)) return fail(ATJ_ERR_INVALID, "unknown launch option");
+  // per-wave scratch: the same size for one 64-lane group or two 32-lane groups
+  static_assert(sizeof(GroupScratch<kWave>) == 2 * sizeof(GroupScratch<32>), "group scratch size");
   const size_t ntp = (size_t)a.nrt * a.nlt;
   const size_t base =
-      (ntp + 1) * sizeof(float4) + wpb * sizeof(WaveScratch) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+      (ntp + 1) * sizeof(float4) + wpb * sizeof(GroupScratch<kWave>) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
-  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  const bool rec_global = launch && (launch->options & ATJ_LAUNCH_REC_GLOBAL);
-  if (!rec_global && with_rec <= d.lds_per_block)
-    return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
-  if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
-  return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
+  const bool rec_global = (options & ATJ_LAUNCH_REC_GLOBAL) != 0;
+  if (options & ATJ_LAUNCH_GROUP32) return launch_paths<32, MODE>(a, wpb, base, rec_global, blocks, d, stream);
+  return launch_paths<kWave, MODE>(a, wpb, base, rec_global, blocks, d, stream);
 }
 
 }  // namespace

@@ -120,6 +120,9 @@ typedef struct AtjPoseState {
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
 #define ATJ_LAUNCH_REC_GLOBAL 0x1u  /* read receptor beads through the vector-memory (L1) path, not LDS */
+#define ATJ_LAUNCH_GROUP32 0x2u     /* two poses per wavefront (32 lanes each) instead of one.  Results
+                                       differ from the one-pose-per-wave form in rounding only (summation
+                                       order); a pose's result does not depend on the other poses. */
 typedef struct AtjLaunch {
   int32_t waves_per_block;  /* 1..16 (0 = default 4) */
   int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
