@@ -1,6 +1,6 @@
 """Launch-shape autotuner of the persistent pose kernels.
 
-Times a short persistent minimization (a few ADAM steps of a pose sample) for each
+Times a short persistent minimization for each
 candidate (waves per block, receptor in LDS or read through L1, one pose per wavefront or
 two) and keeps the fastest; results are cached per process by (receptor size, grid dims,
 ligand-size histogram bucket, candidate set)."""
@@ -17,7 +17,11 @@ CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _L1), (8, _L1), (8, _G32), (16, _G32
 _cache = {}
 
 
-def autotune_minimize(system, plan, w, device, candidates=CANDIDATES, steps=5, sample=32768, log=None):
+def autotune_minimize(system, plan, w, device, candidates=CANDIDATES, steps=25, sample=131072, log=None):
+    """Steady-state timing: `steps` ADAM steps (at most the plan's) so that per-step work,
+    not per-pose set-up, decides; `sample` poses so that each wavefront gets about 32 of
+    them at 16 waves per CU and the persistent kernel's tail stays small."""
+    steps = max(1, min(steps, plan.nsteps))
     nb = sorted(l.nbeads for l in plan.ligands)
     key = (system.receptor.nrec, tuple(system.grid.dim), nb[len(nb) // 2] // 8, tuple(candidates))
     if key in _cache:

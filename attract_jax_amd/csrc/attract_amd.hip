@@ -213,6 +213,20 @@ __device__ __forceinline__ void group_sum_all(float acc[kAcc], int lane) {
 }
 
 // Number of lanes of the caller's group with `pred` set (scalar bit counts).
+// Inclusive prefix maximum over the group (row shifts, then row broadcasts).  Lanes
+// without a source lane take the maximum with 0, which callers tolerate (values >= -1,
+// combined with a non-negative carry).
+template <int GS>
+__device__ __forceinline__ int group_max_scan(int v) {
+  v = max(v, dppi<0x111>(v));
+  v = max(v, dppi<0x112>(v));
+  v = max(v, dppi<0x114>(v));
+  v = max(v, dppi<0x118>(v));
+  v = max(v, dppi<0x142, 0xA>(v));
+  if (GS == kWave) v = max(v, dppi<0x143, 0xC>(v));
+  return v;
+}
+
 template <int GS>
 __device__ __forceinline__ unsigned group_count(bool pred, int lane) {
   const unsigned long long b = __builtin_amdgcn_ballot_w64(pred);
@@ -382,22 +396,46 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, ivor * rmin2, (ivor - 1) emin): ivor = +-1 rides on the sign
-// of rmin2 (ATTRACT's ipon is +-1, attract-original/convert-attract-par.py:14-15).  An ivor
-// of another magnitude poisons the record (NaN energies) rather than being misread.
-__device__ __forceinline__ float4 pair_record(float rc, float ac, float iv, float em, float r2) {
+// 16-byte pair record (rc, ac, ivor * rmin2, c_lo) and 8-byte side record (c_hi, gP) of
+// one pair type.  ivor = +-1 rides on the sign of rmin2 (ATTRACT's ipon is +-1,
+// attract-original/convert-attract-par.py:14-15); an ivor of another magnitude poisons
+// the record (NaN energies) rather than being misread.  The pair energy is
+// s vlj + (dsq < rmin2 ? c_lo : c_hi) and its gradient factor s dv2 + gP / sqrt(dsq), so
+// everything that depends on the pair type only is folded in here: c_lo = (ivor - 1) emin
+// and c_hi = 0 without the plateau; with it both carry -E(plateaudis) and gP the
+// force-shift factor -2 P dE/ddsq (P) (test-grid-energrad.py:124-147), zero for the exact
+// derivative.
+struct PairType {
+  float4 rec;
+  float2 side;
+};
+
+__device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, float em, float r2, bool plateau,
+                                              bool exact, const PlateauConst& pc) {
   if (fabsf(iv) != 1.f) rc = __int_as_float(0x7fc00000);
-  return make_float4(rc, ac, iv < 0.f ? -r2 : r2, (iv - 1.f) * em);
+  const float w = (iv - 1.f) * em;
+  PairType t;
+  t.rec = make_float4(rc, ac, iv < 0.f ? -r2 : r2, w);
+  t.side = make_float2(0.f, 0.f);
+  if (plateau) {
+    const float vljP = fmaf(rc, pc.uP4, -(ac * pc.uP3));
+    const bool loP = pc.P2 < r2;
+    const float sP = loP ? 1.f : iv;
+    const float EP = fmaf(sP, vljP, loP ? w : 0.f);
+    t.rec.w = w - EP;
+    t.side.x = -EP;
+    if (!exact) t.side.y = -(sP * fmaf(rc, pc.k1, ac * pc.k2));
+  }
+  return t;
 }
 
 // ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and 2 x its dsq-derivative,
-// optionally minus the plateau energy with the force-shifted gradient (:124-147), the
-// plateau terms evaluated from the record at P2.  Returns E and gs such that
-// dE/dx'_lig = gs * (x'_lig - x_rec).  The all-zero record gives E = gs = 0 for any
-// finite dsq > 0 (padding and pairs beyond the plateau).
+// with the pair-type terms of the record (plateau subtraction and force shift when the
+// record carries them).  Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).
+// The all-zero record gives E = gs = 0 for any finite dsq > 0 (padding and pairs beyond
+// the plateau).
 template <bool PLATEAU, bool EXACT>
-__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const PlateauConst& pc, float& E,
-                                          float& gs) {
+__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& gs) {
   float u, rs = 0.f;
   if (PLATEAU && !EXACT) {
     rs = __builtin_amdgcn_rsqf(dsq);
@@ -415,26 +453,20 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const Plat
   const float iv = pa.z < 0.f ? -1.f : 1.f;
   const bool lo = dsq < r2;
   const float s = lo ? 1.f : iv;
-  E = fmaf(s, vlj, lo ? pa.w : 0.f);
+  E = fmaf(s, vlj, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
   gs = s * dv2;
-  if (PLATEAU) {
-    const float vljP = fmaf(pa.x, pc.uP4, -(pa.y * pc.uP3));
-    const bool loP = pc.P2 < r2;
-    const float sP = loP ? 1.f : iv;
-    E -= fmaf(sP, vljP, loP ? pa.w : 0.f);
-    if (!EXACT) gs = fmaf(-(sP * fmaf(pa.x, pc.k1, pa.y * pc.k2)), rs, gs);
-  }
+  if (PLATEAU && !EXACT) gs = fmaf(ps.y, rs, gs);
 }
 
-__device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const float4 pa,
-                                             const PlateauConst& pc, float& E, float& gs) {
+__device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const PairType& t, float& E,
+                                             float& gs) {
   if (plateau) {
     if (exact)
-      pair_eval<true, true>(dsq, pa, pc, E, gs);
+      pair_eval<true, true>(dsq, t.rec, t.side, E, gs);
     else
-      pair_eval<true, false>(dsq, pa, pc, E, gs);
+      pair_eval<true, false>(dsq, t.rec, t.side, E, gs);
   } else {
-    pair_eval<false, false>(dsq, pa, pc, E, gs);
+    pair_eval<false, false>(dsq, t.rec, t.side, E, gs);
   }
 }
 
@@ -450,11 +482,20 @@ __device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, floa
   d[2] = lo ? (iv - 1.f) : 0.f;
 }
 
+// LDS pair-table rows (one per receptor type) have an odd stride of nlt | 1 records, so
+// the records of one ligand type under different receptor types fall in different LDS
+// banks (lanes working on one ligand bead read one column).  Slot nslots holds the
+// all-zero record.
+__host__ __device__ __forceinline__ int pair_stride(int nlt) { return nlt | 1; }
+__host__ __device__ __forceinline__ int pair_slots(int nrt, int nlt) { return nrt * pair_stride(nlt); }
+// Side-record slots: nslots + 1 rounded up to even, keeping what follows 16-byte aligned.
+__host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots + 2) & ~1; }
+
 template <int GS>
 struct GroupScratch {
-  float4 xp[GS];   // x', y', z', ligand ff type (int bits)
-  float4 xl[GS];   // x, y, z (local), list start (int bits)
-  int segincl[GS]; // inclusive scan of list segments per bead
+  float4 xp[GS];  // x', y', z', ligand ff type (int bits)
+  float4 xl[GS];  // x, y, z (local), list start - S * first item (int bits)
+  int mark[GS];   // bead whose first list segment is item base + i of the current window
 };
 
 struct KArgs {
@@ -494,30 +535,38 @@ struct KArgs {
 __device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
 
 // Pair loop over the flattened list segments (S entries each) of one chunk of beads.
+// Bead lane j brings its segment count `segs` and first item `excl`; each window of GS
+// items finds its beads by marking the segment starts that fall in the window and taking
+// a prefix maximum (the bead of the window's first item is carried over).
 template <int GS, int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
-                                           const float4* __restrict__ s_par, float* s_pgrad,
-                                           const GroupScratch<GS>* ws, int total, float wgt, float acc[kAcc],
-                                           unsigned& npair, int lane) {
+                                           const float4* __restrict__ s_par, const float2* __restrict__ s_side,
+                                           float* s_pgrad, GroupScratch<GS>* ws, int total, int segs, int excl,
+                                           float wgt, float acc[kAcc], unsigned& npair, int lane) {
   const int gl = lane & (GS - 1);
   const GridView& G = a.G;
   const float P2 = G.p2;
-  const PlateauConst pc = plateau_const(P2);
   const unsigned nrec = (unsigned)a.nrec;
   const unsigned nlt = (unsigned)a.nlt;
   const unsigned ntp = (unsigned)(a.nrt * a.nlt);
-  int jb = 0;
+  const unsigned stride = (unsigned)pair_stride(a.nlt);
+  const unsigned nslots = (unsigned)pair_slots(a.nrt, a.nlt);
+  int carry = 0;
   for (int base = 0; base < total; base += GS) {
+    ws->mark[gl] = -1;
+    wave_sync();
+    const int d = excl - base;
+    if (segs > 0 && d >= 0 && d < GS) ws->mark[d] = gl;
+    wave_sync();
+    const int jb = max(group_max_scan<GS>(ws->mark[gl]), carry);
+    carry = group_value_i<GS>(jb, GS - 1, lane);
     const int item = base + gl;
     const bool act = item < total;
-    if (act)
-      while (ws->segincl[jb] <= item) ++jb;
-    const int sbase = jb ? ws->segincl[jb - 1] : 0;
     const float4 bx = ws->xp[jb];
     const float4 bl = ws->xl[jb];
     const unsigned blt = (unsigned)__float_as_int(bx.w);
     // inactive lanes read the all-padding segment at the head of the list array
-    const int off = act ? __float_as_int(bl.w) + (item - sbase) * S : 0;
+    const int off = act ? __float_as_int(bl.w) + item * S : 0;
     unsigned words[S / 2];
     if (S == 8) {
       const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);
@@ -547,6 +596,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       }
       float dd[4][3], dsq[4];
       float4 pa[4];
+      float2 ps[4];
       bool ok[4];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
@@ -555,13 +605,14 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         dd[cc][2] = bx.z - rb[cc].z;
         dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
         ok[cc] = dsq[cc] < P2;
-        const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * nlt + blt : ntp;
+        const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * stride + blt : nslots;
         pa[cc] = s_par[pi];
+        ps[cc] = PLATEAU ? s_side[pi] : make_float2(0.f, 0.f);
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         float E, gs;
-        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], pc, E, gs);
+        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, gs);
         e += E;
         gx = fmaf(gs, dd[cc][0], gx);
         gy = fmaf(gs, dd[cc][1], gy);
@@ -600,7 +651,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
 // wave-reduced sums, identical on every lane.
 template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              float* s_pgrad, GroupScratch<GS>* ws, const float4* __restrict__ lig, int nb,
+                              const float2* __restrict__ s_side, float* s_pgrad, GroupScratch<GS>* ws, const float4* __restrict__ lig, int nb,
                               const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
   const int gl = lane & (GS - 1);
@@ -651,23 +702,27 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     if (!do_nb) continue;
     // segment size per chunk: 8 (fewer items) or 4 (less padding / fewer idle lanes)
-    const int incl8 = group_incl_scan<GS>((cnt + 7) >> 3);
-    const int incl4 = group_incl_scan<GS>((cnt + 3) >> 2);
+    const int seg8 = (cnt + 7) >> 3, seg4 = (cnt + 3) >> 2;
+    const int incl8 = group_incl_scan<GS>(seg8);
+    const int incl4 = group_incl_scan<GS>(seg4);
     const int tot8 = group_value_i<GS>(incl8, GS - 1, lane);
     const int tot4 = group_value_i<GS>(incl4, GS - 1, lane);
     if (tot8 == 0) continue;
     const int cost8 = ((tot8 + GS - 1) / GS) * (8 * kPairCost + kItemCost);
     const int cost4 = ((tot4 + GS - 1) / GS) * (4 * kPairCost + kItemCost);
     const bool use8 = cost8 <= cost4;
+    const int segs = use8 ? seg8 : seg4;
+    const int excl = (use8 ? incl8 : incl4) - segs;
     ws->xp[gl] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-    ws->xl[gl] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
-    ws->segincl[gl] = use8 ? incl8 : incl4;
+    ws->xl[gl] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
     wave_sync();
     unsigned npair = 0;  // group-wide count, credited to the group's lane 0
     if (use8)
-      pair_items<GS, 8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot8, wgt, acc, npair, lane);
+      pair_items<GS, 8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot8, segs, excl, wgt,
+                                                       acc, npair, lane);
     else
-      pair_items<GS, 4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_pgrad, ws, tot4, wgt, acc, npair, lane);
+      pair_items<GS, 4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot4, segs, excl, wgt,
+                                                       acc, npair, lane);
     st[3] += gl == 0 ? npair : 0u;
     wave_sync();
   }
@@ -689,18 +744,31 @@ __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float 
   }
 }
 
+
 // Stage the receptor beads (+ the padding sentinel) and the pair table (+ the
 // all-zero record) into LDS.
-__device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float* s_pgrad) {
+__device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* s_side, bool plateau, bool exact,
+                          float* s_pgrad) {
   const int tid = threadIdx.x;
   if (s_rec) {
     for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = a.rec[i];
     if (tid == 0) s_rec[a.nrec] = sentinel_bead();
   }
   const int ntp = a.nrt * a.nlt;
-  for (int i = tid; i < ntp; i += blockDim.x)
-    s_par[i] = pair_record(a.rc[i], a.ac[i], a.ivor[i], a.emin[i], a.rmin2[i]);
-  if (tid == 0) s_par[ntp] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
+  const PlateauConst pc = plateau_const(a.G.p2);
+  for (int i = tid; i <= nslots; i += blockDim.x) {
+    const int rt = i / stride, lt = i - rt * stride;
+    PairType t;
+    t.rec = make_float4(0.f, 0.f, 0.f, 0.f);
+    t.side = make_float2(0.f, 0.f);
+    if (i < nslots && lt < a.nlt) {
+      const int k = rt * a.nlt + lt;
+      t = pair_type(a.rc[k], a.ac[k], a.ivor[k], a.emin[k], a.rmin2[k], plateau, exact, pc);
+    }
+    s_par[i] = t.rec;
+    s_side[i] = t.side;
+  }
   if (s_pgrad)
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
@@ -714,12 +782,16 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int lane = threadIdx.x % kWave;
   const int gl = lane & (GS - 1);
   const int ntp = a.nrt * a.nlt;
+  const int nslots = pair_slots(a.nrt, a.nlt);
+  // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
+  // group scratch | pair-table gradient (adjoint, [3][ntp])
   float4* s_par = smem;
-  float4* s_rec = s_par + (ntp + 1);
+  float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
+  float4* s_rec = reinterpret_cast<float4*>(s_side + side_slots(nslots));
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   GroupScratch<GS>* s_ws = reinterpret_cast<GroupScratch<GS>*>(s_rec + nrec_lds);
   float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves * kGroups) : nullptr;
-  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_pgrad);
+  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad);
   __syncthreads();
   GroupScratch<GS>* ws = s_ws + wave * kGroups + lane / GS;
   const bool qt = (a.flags & ATJ_POSE_QT) != 0;
@@ -764,7 +836,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
         quat_to_mat(q, R);
         const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
                              R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
-        pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, nullptr, ws, lig, nb, M, 0.f, acc, st,
+        pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws, lig, nb, M, 0.f, acc, st,
                                                          lane);
         elast = acc[0];
         float A[9];
@@ -813,7 +885,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       float M[16];
       load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
       const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
+      pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
       if (gl == 0) {
         a.energy[p] = acc[0];
         if (a.dEdM) {
@@ -897,9 +969,10 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
       if (dsq < a.G.p2) {
         const int rt = __float_as_int(rb.w);
         const int pi = rt * a.nlt + lt;
-        const float4 pa = pair_record(a.rc[pi], a.ac[pi], a.ivor[pi], a.emin[pi], a.rmin2[pi]);
+        const PairType t =
+            pair_type(a.rc[pi], a.ac[pi], a.ivor[pi], a.emin[pi], a.rmin2[pi], plateau, exact, plateau_const(a.G.p2));
         float Ep, gs;
-        pair_eval_rt(plateau, exact, dsq, pa, plateau_const(a.G.p2), Ep, gs);
+        pair_eval_rt(plateau, exact, dsq, t, Ep, gs);
         e += Ep;
         gg[0] = fmaf(gs, ddx, gg[0]);
         gg[1] = fmaf(gs, ddy, gg[1]);
@@ -1237,8 +1310,9 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   // per-wave scratch: the same size for one 64-lane group or two 32-lane groups
   static_assert(sizeof(GroupScratch<kWave>) == 2 * sizeof(GroupScratch<32>), "group scratch size");
   const size_t ntp = (size_t)a.nrt * a.nlt;
-  const size_t base =
-      (ntp + 1) * sizeof(float4) + wpb * sizeof(GroupScratch<kWave>) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+  const int nslots = pair_slots(a.nrt, a.nlt);
+  const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
+                      wpb * sizeof(GroupScratch<kWave>) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
   const bool rec_global = (options & ATJ_LAUNCH_REC_GLOBAL) != 0;
