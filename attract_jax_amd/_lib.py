@@ -138,10 +138,11 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    path = os.environ.get("ATTRACT_AMD_LIB", LIB_PATH)  # another build of the same library
+    if not os.path.exists(path):
         raise AttractError(
-            f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
-    lib = ctypes.CDLL(LIB_PATH)
+            f"{path} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
     for name, (argtypes, restype) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes

@@ -21,6 +21,10 @@ import torch
 from . import _lib
 
 SEG = 8
+# Non-empty lists start on a 64-byte boundary (32 entries), so a list of up to 32 entries
+# is one memory line and a longer one the fewest lines; the head of the list array is
+# one such line of padding (read by idle lanes).
+LINE = 32
 
 
 def _ptr(t):
@@ -124,18 +128,18 @@ class ForceField:
 
 def lists_from_csr(counts, starts0, flat):
     """Device neighbour-list format from CSR lists (counts, 0-based starts, flat indices):
-    returns (nbgrid (V,2) int32, nblist uint16) with 8-aligned starts >= 8, 0xFFFF padding
-    and an all-padding first segment."""
+    returns (nbgrid (V,2) int32, nblist uint16) with 0xFFFF padding to a multiple of 8
+    entries, non-empty lists starting at multiples of 32 entries (>= 32) and an
+    all-padding head of 32 entries."""
     counts = np.asarray(counts, np.int64).reshape(-1)
     counts = np.maximum(counts, 0)
     starts0 = np.asarray(starts0, np.int64).reshape(-1)
     flat = np.asarray(flat)
     if len(flat) and int(flat.max()) >= 0xFFFF:
         raise ValueError("neighbour lists need receptor indices < 65535")
-    padded = (counts + SEG - 1) // SEG * SEG
-    # the first segment is all padding (read by idle lanes); lists follow it
-    new_starts = SEG + np.concatenate(([0], np.cumsum(padded)[:-1]))
-    total = SEG + int(padded.sum())
+    slot = np.where(counts > 0, (counts + LINE - 1) // LINE * LINE, 0)
+    new_starts = LINE + np.concatenate(([0], np.cumsum(slot)[:-1]))
+    total = LINE + int(slot.sum())
     nblist = np.full(total, 0xFFFF, np.uint16)
     n = int(counts.sum())
     if n:
@@ -296,12 +300,13 @@ class DeviceGrid:
         counts = torch.empty(V, dtype=torch.int32, device=d)
         _lib.check(lib.atj_grid_count_neighbours(ctypes.byref(geom), ctypes.byref(rstruct),
                                                  float(neighbourdis), _ptr(counts), st))
-        padded = (counts.to(torch.int64) + SEG - 1) // SEG * SEG
-        ends = torch.cumsum(padded, 0) + SEG
-        starts = (ends - padded).contiguous()
-        total = int(ends[-1].item()) if V else SEG
+        c64 = counts.to(torch.int64)
+        slot = torch.where(c64 > 0, (c64 + LINE - 1) // LINE * LINE, torch.zeros_like(c64))
+        ends = torch.cumsum(slot, 0) + LINE
+        starts = (ends - slot).contiguous()
+        total = int(ends[-1].item()) if V else LINE
         self.nblist = torch.empty(total, dtype=torch.int16, device=d)
-        self.nblist[:SEG].fill_(-1)
+        self.nblist[:LINE].fill_(-1)
         self.nbgrid = torch.empty((V, 2), dtype=torch.int32, device=d)
         _lib.check(lib.atj_grid_fill_neighbours(ctypes.byref(self.struct()), ctypes.byref(rstruct),
                                                 float(neighbourdis), _ptr(counts), _ptr(starts),

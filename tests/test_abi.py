@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 
 from attract_jax_amd import _lib
-from attract_jax_amd.device import SEG, lists_from_csr, pack_beads
+from attract_jax_amd.device import LINE, SEG, lists_from_csr, pack_beads
 from tests.conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "attract_amd.h")
@@ -49,7 +49,8 @@ def test_device_list_format():
     nbgrid, nblist = lists_from_csr(counts, starts, flat)
     assert np.array_equal(nbgrid[:, 0], counts)
     assert np.all(nbgrid[:, 1] % SEG == 0) and np.all(nbgrid[:, 1] >= SEG)
-    assert np.all(nblist[:SEG] == 0xFFFF)
+    assert np.all(nbgrid[counts > 0, 1] % LINE == 0)
+    assert np.all(nblist[:LINE] == 0xFFFF)
     for v in range(len(counts)):
         c, s = nbgrid[v]
         assert np.array_equal(nblist[s:s + c], flat[starts[v]:starts[v] + c])
