@@ -88,7 +88,6 @@ class AtjPoseState(ctypes.Structure):
 
 
 ATJ_LAUNCH_REC_GLOBAL = 0x1
-ATJ_LAUNCH_GROUP32 = 0x2
 
 
 class AtjLaunch(ctypes.Structure):

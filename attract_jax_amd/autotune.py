@@ -1,9 +1,8 @@
 """Launch-shape autotuner of the persistent pose kernels.
 
-Times a short persistent minimization for each
-candidate (waves per block, receptor in LDS or read through L1, one pose per wavefront or
-two) and keeps the fastest; results are cached per process by (receptor size, grid dims,
-ligand-size histogram bucket, candidate set)."""
+Times a short persistent minimization for each candidate (waves per block, receptor in
+LDS or read through L1) and keeps the fastest; results are cached per process by
+(receptor size, grid dims, ligand-size histogram bucket, candidate set)."""
 import time
 
 import torch
@@ -11,9 +10,8 @@ import torch
 from . import _lib
 from .minimize import BatchPlan, PoseState, minimize
 
-_G32 = _lib.ATJ_LAUNCH_GROUP32
 _L1 = _lib.ATJ_LAUNCH_REC_GLOBAL
-CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _L1), (8, _L1), (8, _G32), (16, _G32), (16, _G32 | _L1))
+CANDIDATES = ((4, 0), (8, 0), (16, 0), (16, _L1), (8, _L1))
 _cache = {}
 
 

@@ -10,10 +10,10 @@
 //     in 16-byte segments of 8 receptor indices (lane = segment), so lanes stay
 //     busy whatever the per-bead list lengths; receptor beads and the pair table
 //     are read from LDS.
-// Energy and dE/dM (13 floats) are accumulated per lane and reduced with an xor
-// butterfly, which gives every lane the bit-identical sum; the ADAM update of
-// the minimizer then runs redundantly on all lanes with the pose state in
-// registers across steps.
+// Energy and dE/dM (13 floats) are accumulated per lane and reduced together by a
+// transposed butterfly of lane swaps, in a fixed order.  The minimizer gives each
+// wavefront a batch of poses: per ADAM step it evaluates them one after the other,
+// then updates them together, one lane per pose, with the state in LDS.
 //
 // The file is compiled with -ffp-contract=off: every fused multiply-add below
 // is written explicitly, so the CPU twin can pin the operation order of the
@@ -43,11 +43,11 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kWave = 64;
-constexpr int kSeg = 8;            // receptor indices per 16-byte list segment
+constexpr int kListPad = 16;       // lists are padded with 0xFFFF to a multiple of this
 constexpr int kMaxWavesPerBlock = 16;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
-constexpr int kPairCost = 36;      // approximate instructions per neighbour slot
-constexpr int kItemCost = 40;      // approximate instructions per list segment
+constexpr int kPairCost = 30;      // approximate instructions per neighbour slot
+constexpr int kItemCost = 90;      // approximate instructions per list segment
 
 enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2 };
 
@@ -111,64 +111,26 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
-// A pose is owned by a group of GS lanes (GS = 64: one pose per wave; GS = 32: two).
-// Group-uniform values: scalar registers for GS = 64, per-lane copies otherwise.
-template <int GS>
-__device__ __forceinline__ float guni(float v) {
-  if (GS == kWave) return uni(v);
-  return v;
-}
-template <int GS>
-__device__ __forceinline__ int guni(int v) {
-  if (GS == kWave) return uni(v);
-  return v;
-}
-
-// Value of group lane `src` of the caller's group.
-template <int GS>
-__device__ __forceinline__ float group_value(float v, int src, int lane) {
-  if (GS == kWave) return lane_value(v, src);
-  const float a0 = lane_value(v, src), a1 = lane_value(v, 32 + src);
-  return lane < 32 ? a0 : a1;
-}
-template <int GS>
-__device__ __forceinline__ int group_value_i(int v, int src, int lane) {
-  if (GS == kWave) return __builtin_amdgcn_readlane(v, src);
-  const int a0 = __builtin_amdgcn_readlane(v, src), a1 = __builtin_amdgcn_readlane(v, 32 + src);
-  return lane < 32 ? a0 : a1;
-}
-
-// Sum over the group in a fixed order (quad, half-row and row mirrors, row broadcasts).
-template <int GS>
-__device__ __forceinline__ float group_sum(float v, int lane) {
-  if (GS == kWave) return wave_sum(v);
-  v += dpp<0xB1>(v);
-  v += dpp<0x4E>(v);
-  v += dpp<0x141>(v);
-  v += dpp<0x140>(v);
-  v += dpp<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3: the 32-lane sums
-  return group_value<GS>(v, 31, lane);
-}
-
-// Inclusive prefix sum over the group.
-template <int GS>
-__device__ __forceinline__ int group_incl_scan(int v) {
-  if (GS == kWave) return wave_incl_scan(v);
-  v += dppi<0x111>(v);
-  v += dppi<0x112>(v);
-  v += dppi<0x114>(v);
-  v += dppi<0x118>(v);
-  v += dppi<0x142, 0xA>(v);
+// Inclusive prefix maximum over the 64 lanes (row shifts, then row broadcasts).  Lanes
+// without a source lane take the maximum with 0, which callers tolerate (values >= -1,
+// combined with a non-negative carry).
+__device__ __forceinline__ int wave_max_scan(int v) {
+  v = max(v, dppi<0x111>(v));
+  v = max(v, dppi<0x112>(v));
+  v = max(v, dppi<0x114>(v));
+  v = max(v, dppi<0x118>(v));
+  v = max(v, dppi<0x142, 0xA>(v));
+  v = max(v, dppi<0x143, 0xC>(v));
   return v;
 }
 
-// Sums over the group of all kAcc accumulators at once (a transposed butterfly): each
+// Sums over the wave of all kAcc accumulators at once (a transposed butterfly): each
 // halving stage pairs two accumulators, one lane half keeps the pair sum of the first and
 // the other half that of the second, so a pair costs one lane swap and one add.  Stages:
-// lanes l <-> l^32 (GS = 64 only), rows 0 <-> 1 and 2 <-> 3, then in-row mirrors
-// l <-> 15-l and l <-> 7-l, then quad swaps l <-> l^2, l <-> l^1.  Accumulator i ends in the
-// lane whose bits (5, 4, 3, 2) [GS = 64] or (4, 3, 2, 1) [GS = 32] spell out i's bits
-// (0, 1, 2, 3); the order of the additions is fixed, so results are reproducible.
+// lanes l <-> l^32, rows 0 <-> 1 and 2 <-> 3, then in-row mirrors l <-> 15-l and
+// l <-> 7-l, then quad swaps l <-> l^2, l <-> l^1.  The returned value of lane l is the
+// sum of accumulator acc_index(l) (every lane of a quad holds it); the order of the
+// additions is fixed, so results are reproducible.
 __device__ __forceinline__ void swap_add(float a, float b, float& out, bool rows32) {
   const auto r = rows32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
                         : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
@@ -180,59 +142,28 @@ __device__ __forceinline__ float mirror_add(float a, float b, bool upper) {
   return (upper ? b : a) + dpp<CTRL>(upper ? a : b);
 }
 
-template <int GS>
-__device__ __forceinline__ void group_sum_all(float acc[kAcc], int lane) {
+__device__ __forceinline__ float wave_sum_all(const float acc[kAcc], int lane) {
   static_assert(kAcc == 13, "reduction schedule is laid out for 13 accumulators");
   float s1[7], s2[4], s3[2];
-  if (GS == kWave) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) swap_add(acc[2 * k], 2 * k + 1 < kAcc ? acc[2 * k + 1] : 0.f, s1[k], true);
+  for (int k = 0; k < 7; ++k) swap_add(acc[2 * k], 2 * k + 1 < kAcc ? acc[2 * k + 1] : 0.f, s1[k], true);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) swap_add(s1[2 * k], 2 * k + 1 < 7 ? s1[2 * k + 1] : 0.f, s2[k], false);
+  for (int k = 0; k < 4; ++k) swap_add(s1[2 * k], 2 * k + 1 < 7 ? s1[2 * k + 1] : 0.f, s2[k], false);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) s3[k] = mirror_add<0x140>(s2[2 * k], s2[2 * k + 1], lane & 8);  // row_mirror
-    float u = mirror_add<0x141>(s3[0], s3[1], lane & 4);                                        // row_half_mirror
-    u += dpp<0x4E>(u);  // quad_perm [2,3,0,1]
-    u += dpp<0xB1>(u);  // quad_perm [1,0,3,2]
-#pragma unroll
-    for (int i = 0; i < kAcc; ++i)
-      acc[i] = lane_value(u, 32 * (i & 1) + 16 * ((i >> 1) & 1) + 8 * ((i >> 2) & 1) + 4 * (i >> 3));
-  } else {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) swap_add(acc[2 * k], 2 * k + 1 < kAcc ? acc[2 * k + 1] : 0.f, s1[k], false);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) s2[k] = mirror_add<0x140>(s1[2 * k], 2 * k + 1 < 7 ? s1[2 * k + 1] : 0.f, lane & 8);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) s3[k] = mirror_add<0x141>(s2[2 * k], s2[2 * k + 1], lane & 4);
-    float u = mirror_add<0x4E>(s3[0], s3[1], lane & 2);  // quad_perm [2,3,0,1]
-    u += dpp<0xB1>(u);                                   // quad_perm [1,0,3,2]
-#pragma unroll
-    for (int i = 0; i < kAcc; ++i)
-      acc[i] = group_value<GS>(u, 16 * (i & 1) + 8 * ((i >> 1) & 1) + 4 * ((i >> 2) & 1) + 2 * (i >> 3), lane);
-  }
+  for (int k = 0; k < 2; ++k) s3[k] = mirror_add<0x140>(s2[2 * k], s2[2 * k + 1], lane & 8);  // row_mirror
+  float u = mirror_add<0x141>(s3[0], s3[1], lane & 4);                                        // row_half_mirror
+  u += dpp<0x4E>(u);  // quad_perm [2,3,0,1]
+  u += dpp<0xB1>(u);  // quad_perm [1,0,3,2]
+  return u;
 }
 
-// Number of lanes of the caller's group with `pred` set (scalar bit counts).
-// Inclusive prefix maximum over the group (row shifts, then row broadcasts).  Lanes
-// without a source lane take the maximum with 0, which callers tolerate (values >= -1,
-// combined with a non-negative carry).
-template <int GS>
-__device__ __forceinline__ int group_max_scan(int v) {
-  v = max(v, dppi<0x111>(v));
-  v = max(v, dppi<0x112>(v));
-  v = max(v, dppi<0x114>(v));
-  v = max(v, dppi<0x118>(v));
-  v = max(v, dppi<0x142, 0xA>(v));
-  if (GS == kWave) v = max(v, dppi<0x143, 0xC>(v));
-  return v;
+// Accumulator whose sum wave_sum_all leaves in lane l: lane bits (5, 4, 3, 2) are its
+// bits (0, 1, 2, 3).  Values 13..15 are the zero padding.
+__device__ __forceinline__ int acc_index(int l) {
+  return ((l >> 5) & 1) | (((l >> 4) & 1) << 1) | (((l >> 3) & 1) << 2) | (((l >> 2) & 1) << 3);
 }
-
-template <int GS>
-__device__ __forceinline__ unsigned group_count(bool pred, int lane) {
-  const unsigned long long b = __builtin_amdgcn_ballot_w64(pred);
-  if (GS == kWave) return (unsigned)__builtin_popcountll(b);
-  const unsigned c0 = (unsigned)__builtin_popcount((unsigned)b), c1 = (unsigned)__builtin_popcount((unsigned)(b >> 32));
-  return lane < 32 ? c0 : c1;
+__host__ __device__ constexpr int acc_lane(int i) {
+  return 32 * (i & 1) + 16 * ((i >> 1) & 1) + 8 * ((i >> 2) & 1) + 4 * (i >> 3);
 }
 
 // Rotation matrix of a (not necessarily normalised) quaternion (x, y, z, w):
@@ -491,11 +422,22 @@ __host__ __device__ __forceinline__ int pair_slots(int nrt, int nlt) { return nr
 // Side-record slots: nslots + 1 rounded up to even, keeping what follows 16-byte aligned.
 __host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots + 2) & ~1; }
 
-template <int GS>
-struct GroupScratch {
-  float4 xp[GS];  // x', y', z', ligand ff type (int bits)
-  float4 xl[GS];  // x, y, z (local), list start - S * first item (int bits)
-  int mark[GS];   // bead whose first list segment is item base + i of the current window
+struct WaveScratch {
+  float4 xp[kWave];  // x', y', z', ligand ff type (int bits)
+  float4 xl[kWave];  // x, y, z (local), list start - S * first item (int bits)
+  int mark[kWave];   // bead whose first list segment is item base + i of the current window
+};
+
+// Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
+// evaluates one after the other, and whose ADAM updates it then runs together (lane =
+// slot).
+constexpr int kBatch = 8;
+struct BatchScratch {
+  float M[kBatch][12];    // rotation rows and translation for the next evaluation
+  float acc[kBatch][16];  // reduced sums of the last evaluation (kAcc used)
+  float prm[kBatch][7];   // quaternion (x, y, z, w), translation
+  float m1[kBatch][7];    // ADAM moments
+  float m2[kBatch][7];
 };
 
 struct KArgs {
@@ -535,15 +477,14 @@ struct KArgs {
 __device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
 
 // Pair loop over the flattened list segments (S entries each) of one chunk of beads.
-// Bead lane j brings its segment count `segs` and first item `excl`; each window of GS
+// Bead lane j brings its segment count `segs` and first item `excl`; each window of 64
 // items finds its beads by marking the segment starts that fall in the window and taking
 // a prefix maximum (the bead of the window's first item is carried over).
-template <int GS, int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
                                            const float4* __restrict__ s_par, const float2* __restrict__ s_side,
-                                           float* s_pgrad, GroupScratch<GS>* ws, int total, int segs, int excl,
+                                           float* s_pgrad, WaveScratch* ws, int total, int segs, int excl,
                                            float wgt, float acc[kAcc], unsigned& npair, int lane) {
-  const int gl = lane & (GS - 1);
   const GridView& G = a.G;
   const float P2 = G.p2;
   const unsigned nrec = (unsigned)a.nrec;
@@ -552,21 +493,23 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   const unsigned stride = (unsigned)pair_stride(a.nlt);
   const unsigned nslots = (unsigned)pair_slots(a.nrt, a.nlt);
   int carry = 0;
-  for (int base = 0; base < total; base += GS) {
-    ws->mark[gl] = -1;
+  for (int base = 0; base < total; base += kWave) {
+    ws->mark[lane] = -1;
     wave_sync();
     const int d = excl - base;
-    if (segs > 0 && d >= 0 && d < GS) ws->mark[d] = gl;
+    if (segs > 0 && d >= 0 && d < kWave) ws->mark[d] = lane;
     wave_sync();
-    const int jb = max(group_max_scan<GS>(ws->mark[gl]), carry);
-    carry = group_value_i<GS>(jb, GS - 1, lane);
-    const int item = base + gl;
+    const int jb = max(wave_max_scan(ws->mark[lane]), carry);
+    carry = __builtin_amdgcn_readlane(jb, kWave - 1);
+    const int item = base + lane;
     const bool act = item < total;
     const float4 bx = ws->xp[jb];
     const float4 bl = ws->xl[jb];
     const unsigned blt = (unsigned)__float_as_int(bx.w);
     // inactive lanes read the all-padding segment at the head of the list array
     const int off = act ? __float_as_int(bl.w) + item * S : 0;
+    static_assert(S == 4 || S == 8, "segments of 4 or 8 list entries");
+    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     unsigned words[S / 2];
     if (S == 8) {
       const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);
@@ -579,7 +522,6 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       words[0] = ent.x;
       words[1] = ent.y;
     }
-    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
     for (int h = 0; h < S / 4; ++h) {
       float4 rb[4];
@@ -605,7 +547,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         dd[cc][2] = bx.z - rb[cc].z;
         dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
         ok[cc] = dsq[cc] < P2;
-        const unsigned pi = ok[cc] ? (unsigned)__float_as_int(rb[cc].w) * stride + blt : nslots;
+        const unsigned pi = ok[cc] ? __umul24((unsigned)__float_as_int(rb[cc].w), stride) + blt : nslots;
         pa[cc] = s_par[pi];
         ps[cc] = PLATEAU ? s_side[pi] : make_float2(0.f, 0.f);
       }
@@ -617,21 +559,21 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         gx = fmaf(gs, dd[cc][0], gx);
         gy = fmaf(gs, dd[cc][1], gy);
         gz = fmaf(gs, dd[cc][2], gz);
-        npair += group_count<GS>(ok[cc], lane);
+        npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
         if (MODE == kAdjoint && ok[cc]) {
-          float d[3];
-          pair_param_grad(dsq[cc], pa[cc], d);
+          float dp[3];
+          pair_param_grad(dsq[cc], pa[cc], dp);
           if (PLATEAU) {
-            float dp[3];
-            pair_param_grad(P2, pa[cc], dp);
-            d[0] -= dp[0];
-            d[1] -= dp[1];
-            d[2] -= dp[2];
+            float dq[3];
+            pair_param_grad(P2, pa[cc], dq);
+            dp[0] -= dq[0];
+            dp[1] -= dq[1];
+            dp[2] -= dq[2];
           }
-          const unsigned pi = (unsigned)__float_as_int(rb[cc].w) * nlt + blt;
-          atomicAdd(&s_pgrad[pi], wgt * d[0]);
-          atomicAdd(&s_pgrad[ntp + pi], wgt * d[1]);
-          atomicAdd(&s_pgrad[2 * ntp + pi], wgt * d[2]);
+          const unsigned pg = (unsigned)__float_as_int(rb[cc].w) * nlt + blt;
+          atomicAdd(&s_pgrad[pg], wgt * dp[0]);
+          atomicAdd(&s_pgrad[ntp + pg], wgt * dp[1]);
+          atomicAdd(&s_pgrad[2 * ntp + pg], wgt * dp[2]);
         }
       }
     }
@@ -647,25 +589,27 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   }
 }
 
-// Per-pose energy and dE/dM for a uniform pose matrix M; acc[] returns the
-// wave-reduced sums, identical on every lane.
-template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
-__device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              const float2* __restrict__ s_side, float* s_pgrad, GroupScratch<GS>* ws, const float4* __restrict__ lig, int nb,
-                              const float Min[16], float wgt, float acc[kAcc], unsigned st[4], int lane) {
+// Energy and dE/dM of one pose (uniform pose matrix M).  Returns the wave-reduced sums in
+// the transposed layout of wave_sum_all: lane l holds accumulator acc_index(l)
+// (0: energy, 1 + 3k + l: dE/dM[k][l], 10 + l: dE/dM[3][l]).
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
+__device__ float pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
+                               const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws,
+                               const float4* __restrict__ lig, int nb, const float Min[16], float wgt,
+                               unsigned st[4], int lane) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
-  const int gl = lane & (GS - 1);
   float M[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) M[i] = guni<GS>(Min[i]);
+  for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
   const GridView& G = a.G;
+  float acc[kAcc];
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
 
-  for (int c0 = 0; c0 < nb; c0 += GS) {
-    const int j = c0 + gl;
+  for (int c0 = 0; c0 < nb; c0 += kWave) {
+    const int j = c0 + lane;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
     int lt = 0;
@@ -701,32 +645,80 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       }
     }
     if (!do_nb) continue;
-    // segment size per chunk: 8 (fewer items) or 4 (less padding / fewer idle lanes)
+    // segment size per chunk: 8 list entries (fewer windows) or 4 (fewer padding slots)
     const int seg8 = (cnt + 7) >> 3, seg4 = (cnt + 3) >> 2;
-    const int incl8 = group_incl_scan<GS>(seg8);
-    const int incl4 = group_incl_scan<GS>(seg4);
-    const int tot8 = group_value_i<GS>(incl8, GS - 1, lane);
-    const int tot4 = group_value_i<GS>(incl4, GS - 1, lane);
+    const int incl8 = wave_incl_scan(seg8);
+    const int incl4 = wave_incl_scan(seg4);
+    const int tot8 = __builtin_amdgcn_readlane(incl8, kWave - 1);
+    const int tot4 = __builtin_amdgcn_readlane(incl4, kWave - 1);
     if (tot8 == 0) continue;
-    const int cost8 = ((tot8 + GS - 1) / GS) * (8 * kPairCost + kItemCost);
-    const int cost4 = ((tot4 + GS - 1) / GS) * (4 * kPairCost + kItemCost);
+    const int cost8 = ((tot8 + kWave - 1) / kWave) * (8 * kPairCost + kItemCost);
+    const int cost4 = ((tot4 + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
     const bool use8 = cost8 <= cost4;
     const int segs = use8 ? seg8 : seg4;
     const int excl = (use8 ? incl8 : incl4) - segs;
-    ws->xp[gl] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-    ws->xl[gl] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
+    ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
+    ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
     wave_sync();
-    unsigned npair = 0;  // group-wide count, credited to the group's lane 0
+    unsigned npair = 0;  // wave-wide count, credited to lane 0
     if (use8)
-      pair_items<GS, 8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot8, segs, excl, wgt,
-                                                       acc, npair, lane);
+      pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot8, segs, excl, wgt, acc,
+                                                   npair, lane);
     else
-      pair_items<GS, 4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot4, segs, excl, wgt,
-                                                       acc, npair, lane);
-    st[3] += gl == 0 ? npair : 0u;
+      pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot4, segs, excl, wgt, acc,
+                                                   npair, lane);
+    st[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
-  group_sum_all<GS>(acc, lane);
+  return wave_sum_all(acc, lane);
+}
+
+// Energy and dE/dM of pose p from the transposed sums: one lane per output word.
+__device__ __forceinline__ void write_energrad(const KArgs& a, int p, float u, int lane) {
+  if (lane & 3) return;
+  const int i = acc_index(lane);
+  if (i == 0) a.energy[p] = u;
+  if (!a.dEdM) return;
+  float* o = a.dEdM + (size_t)p * 16;
+  if (i == 0) o[15] = 0.f;
+  else if (i <= 9) o[4 * ((i - 1) / 3) + (i - 1) % 3] = u;
+  else if (i <= 12) o[12 + (i - 10)] = u;
+  else o[4 * (i - 13) + 3] = 0.f;  // i = 13..15: the zero column of rows 0..2
+}
+
+// One ADAM step of a minimizer slot (optax.adam: bias-corrected moments, eps outside the
+// sqrt, then -lr; test4.py:445-455) followed by Rotation.from_quat(as_quat())
+// renormalisation, with the same operations in the same order as adam_step_kernel.
+__device__ __forceinline__ void adam_update(float prm[7], float m1[7], float m2[7], const float acc[kAcc], float c1,
+                                            float c2, float lr_rot, float lr_trans, float b1, float b2, float eps) {
+  const float q[4] = {prm[0], prm[1], prm[2], prm[3]};
+  const float A[9] = {acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7], acc[8], acc[9]};
+  float gr[7];
+  quat_grad(q, A, gr);
+  gr[4] = acc[10];
+  gr[5] = acc[11];
+  gr[6] = acc[12];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    m1[i] = (1.f - b1) * gr[i] + b1 * m1[i];
+    m2[i] = (1.f - b2) * (gr[i] * gr[i]) + b2 * m2[i];
+    const float upd = (m1[i] / c1) / (sqrtf(m2[i] / c2) + eps);
+    prm[i] = prm[i] + (-(i < 4 ? lr_rot : lr_trans)) * upd;
+  }
+  const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) prm[i] = prm[i] / nrm;
+}
+
+// Rotation rows and translation (12 floats) of a (quaternion, translation) pose.
+__device__ __forceinline__ void pose12(const float prm[7], float M[12]) {
+  float R[9];
+  quat_to_mat(prm, R);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = R[i];
+  M[9] = prm[4];
+  M[10] = prm[5];
+  M[11] = prm[6];
 }
 
 __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float M[16]) {
@@ -773,135 +765,149 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
     for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
-template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
+// Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
+// processing order at a time; per ADAM step it evaluates them one after the other (all
+// 64 lanes on one pose), then lane b runs the ADAM update of slot b.  Slots may belong to
+// different jobs and have different step counts.
+template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
+__device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
+                              const float2* __restrict__ s_side, WaveScratch* ws, BatchScratch* bs, unsigned st[4],
+                              int lane) {
+  for (;;) {
+    int k0 = 0;
+    if (lane == 0) k0 = atomicAdd(a.counter, kBatch);
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    if (k0 >= a.N) break;
+    const int nbatch = min(kBatch, a.N - k0);
+    const bool own = lane < nbatch;
+    int p = 0, jid = 0, cnt = 0, nst = 0;
+    float c1 = 0.f, c2 = 0.f, elast = 0.f;
+    if (own) {
+      const int k = k0 + lane;
+      p = a.pose_order ? a.pose_order[k] : k;
+      jid = a.pose_job ? a.pose_job[p] : 0;
+      const AtjJob job = a.jobs[jid];
+      cnt = a.count[p];
+      elast = a.elast[p];
+      nst = max(0, min(a.nsteps_launch, job.nsteps - cnt));
+      // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
+      for (int i = 0; i < cnt; ++i) {
+        c1 = fmaf(job.beta1, c1, 1.f - job.beta1);
+        c2 = fmaf(job.beta2, c2, 1.f - job.beta2);
+      }
+      float prm[7], M[12];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        prm[i] = a.pose7[7 * p + i];
+        bs->prm[lane][i] = prm[i];
+        bs->m1[lane][i] = a.mom1[7 * p + i];
+        bs->m2[lane][i] = a.mom2[7 * p + i];
+      }
+      pose12(prm, M);
+#pragma unroll
+      for (int i = 0; i < 12; ++i) bs->M[lane][i] = M[i];
+    }
+    int nst_max = 0;
+    for (int b = 0; b < nbatch; ++b) nst_max = max(nst_max, __builtin_amdgcn_readlane(nst, b));
+    wave_sync();
+    for (int s = 0; s < nst_max; ++s) {
+      for (int b = 0; b < nbatch; ++b) {
+        if (s >= __builtin_amdgcn_readlane(nst, b)) continue;
+        const AtjJob job = a.jobs[__builtin_amdgcn_readlane(jid, b)];
+        const float* m = bs->M[b];
+        const float M[16] = {m[0], m[1], m[2], 1.f, m[3], m[4], m[5], 1.f, m[6], m[7], m[8], 1.f,
+                             m[9], m[10], m[11], 1.f};
+        const float u =
+            pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws,
+                                                                      reinterpret_cast<const float4*>(job.ligand.beads),
+                                                                      job.ligand.nbeads, M, 0.f, st, lane);
+        const int i = acc_index(lane);
+        if (!(lane & 3) && i < kAcc) bs->acc[b][i] = u;
+      }
+      wave_sync();
+      if (own && s < nst) {
+        const AtjJob job = a.jobs[jid];
+        float prm[7], m1[7], m2[7], acc[kAcc], M[12];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          prm[i] = bs->prm[lane][i];
+          m1[i] = bs->m1[lane][i];
+          m2[i] = bs->m2[lane][i];
+        }
+#pragma unroll
+        for (int i = 0; i < kAcc; ++i) acc[i] = bs->acc[lane][i];
+        elast = acc[0];
+        ++cnt;
+        c1 = fmaf(job.beta1, c1, 1.f - job.beta1);
+        c2 = fmaf(job.beta2, c2, 1.f - job.beta2);
+        adam_update(prm, m1, m2, acc, c1, c2, job.lr_rot, job.lr_trans, job.beta1, job.beta2, job.eps);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          bs->prm[lane][i] = prm[i];
+          bs->m1[lane][i] = m1[i];
+          bs->m2[lane][i] = m2[i];
+        }
+        pose12(prm, M);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) bs->M[lane][i] = M[i];
+      }
+      wave_sync();
+    }
+    if (own) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        a.pose7[7 * p + i] = bs->prm[lane][i];
+        a.mom1[7 * p + i] = bs->m1[lane][i];
+        a.mom2[7 * p + i] = bs->m2[lane][i];
+      }
+      a.count[p] = cnt;
+      a.elast[p] = elast;
+    }
+    wave_sync();
+  }
+}
+
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
-  constexpr int kGroups = kWave / GS;  // poses in flight per wave
   const int nwaves = blockDim.x / kWave;
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const int gl = lane & (GS - 1);
   const int ntp = a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
-  // group scratch | pair-table gradient (adjoint, [3][ntp])
+  // wave scratch [nwaves] | minimizer batches [nwaves] or pair-table gradient (adjoint, [3][ntp])
   float4* s_par = smem;
   float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
   float4* s_rec = reinterpret_cast<float4*>(s_side + side_slots(nslots));
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
-  GroupScratch<GS>* s_ws = reinterpret_cast<GroupScratch<GS>*>(s_rec + nrec_lds);
-  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves * kGroups) : nullptr;
+  WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
+  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
   stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad);
   __syncthreads();
-  GroupScratch<GS>* ws = s_ws + wave * kGroups + lane / GS;
-  const bool qt = (a.flags & ATJ_POSE_QT) != 0;
+  WaveScratch* ws = s_ws + wave;
   unsigned st[4] = {0, 0, 0, 0};
 
-  for (;;) {
-    int k = 0;
-    if (gl == 0) k = atomicAdd(a.counter, 1);
-    k = GS == kWave ? __builtin_amdgcn_readfirstlane(k) : group_value_i<GS>(k, 0, lane);
-    if (k >= a.N) break;
-    const int p = a.pose_order ? a.pose_order[k] : k;
-    const int jid = a.pose_job ? a.pose_job[p] : 0;
-    const AtjJob job = a.jobs[jid];
-    const float4* lig = reinterpret_cast<const float4*>(job.ligand.beads);
-    const int nb = job.ligand.nbeads;
-    float acc[kAcc];
-    if (MODE == kMinimize) {
-      // Group lane i < 7 owns parameter i of (qx, qy, qz, qw, tx, ty, tz) and its ADAM
-      // moments; the pose itself is uniform over the group.
-      const int pi = gl < 7 ? gl : 6;
-      float prm = a.pose7[7 * p + pi];
-      float m1 = a.mom1[7 * p + pi];
-      float m2 = a.mom2[7 * p + pi];
-      int cnt = guni<GS>(a.count[p]);
-      float elast = a.elast[p];
-      const float lr = gl < 4 ? job.lr_rot : job.lr_trans;
-      const float b1 = job.beta1, b2 = job.beta2;
-      // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
-      float c1 = 0.f, c2 = 0.f;
-      for (int i = 0; i < cnt; ++i) {
-        c1 = fmaf(b1, c1, 1.f - b1);
-        c2 = fmaf(b2, c2, 1.f - b2);
-      }
-      float q[4], t[3];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) q[i] = group_value<GS>(prm, i, lane);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) t[i] = group_value<GS>(prm, 4 + i, lane);
-      const int nst = min(a.nsteps_launch, job.nsteps - cnt);
-      for (int s = 0; s < nst; ++s) {
-        float R[9];
-        quat_to_mat(q, R);
-        const float M[16] = {R[0], R[1], R[2], 1.f, R[3], R[4], R[5], 1.f,
-                             R[6], R[7], R[8], 1.f, t[0], t[1], t[2], 1.f};
-        pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws, lig, nb, M, 0.f, acc, st,
-                                                         lane);
-        elast = acc[0];
-        float A[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) A[i] = acc[1 + i];
-        float gq[4];
-        quat_grad(q, A, gq);
-        float g = acc[12];
-        g = gl == 5 ? acc[11] : g;
-        g = gl == 4 ? acc[10] : g;
-        g = gl == 3 ? gq[3] : g;
-        g = gl == 2 ? gq[2] : g;
-        g = gl == 1 ? gq[1] : g;
-        g = gl == 0 ? gq[0] : g;
-        ++cnt;
-        // optax.adam: scale_by_adam (bias-corrected, eps outside the sqrt) then -lr
-        c1 = guni<GS>(fmaf(b1, c1, 1.f - b1));
-        c2 = guni<GS>(fmaf(b2, c2, 1.f - b2));
-        m1 = (1.f - b1) * g + b1 * m1;
-        m2 = (1.f - b2) * (g * g) + b2 * m2;
-        const float upd = (m1 / c1) / (sqrtf(m2 / c2) + job.eps);
-        prm = prm + (-lr) * upd;
-        // Rotation.from_quat(as_quat()): renormalise the quaternion (lanes 0-3 each divide
-        // their own component)
-        float qn[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) qn[i] = group_value<GS>(prm, i, lane);
-        const float nrm = sqrtf(fmaf(qn[3], qn[3], fmaf(qn[2], qn[2], fmaf(qn[1], qn[1], qn[0] * qn[0]))));
-        const float pn = prm / nrm;
-        prm = gl < 4 ? pn : prm;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = group_value<GS>(prm, i, lane);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) t[i] = group_value<GS>(prm, 4 + i, lane);
-      }
-      if (gl < 7) {
-        a.pose7[7 * p + gl] = prm;
-        a.mom1[7 * p + gl] = m1;
-        a.mom2[7 * p + gl] = m2;
-      }
-      if (gl == 0) {
-        a.count[p] = cnt;
-        a.elast[p] = elast;
-      }
-    } else {
+  if (MODE == kMinimize) {
+    BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
+    minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, ws, bs, st, lane);
+  } else {
+    const bool qt = (a.flags & ATJ_POSE_QT) != 0;
+    for (;;) {
+      int k = 0;
+      if (lane == 0) k = atomicAdd(a.counter, 1);
+      k = __builtin_amdgcn_readfirstlane(k);
+      if (k >= a.N) break;
+      const int p = a.pose_order ? a.pose_order[k] : k;
+      const int jid = a.pose_job ? a.pose_job[p] : 0;
+      const AtjJob job = a.jobs[jid];
       float M[16];
       load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
       const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      pose_energrad<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, lig, nb, M, w, acc, st, lane);
-      if (gl == 0) {
-        a.energy[p] = acc[0];
-        if (a.dEdM) {
-          float* o = a.dEdM + (size_t)p * 16;
-#pragma unroll
-          for (int k2 = 0; k2 < 3; ++k2) {
-#pragma unroll
-            for (int l = 0; l < 3; ++l) o[4 * k2 + l] = acc[1 + 3 * k2 + l];
-            o[4 * k2 + 3] = 0.f;
-          }
-          o[12] = acc[10];
-          o[13] = acc[11];
-          o[14] = acc[12];
-          o[15] = 0.f;
-        }
-      }
+      const float u = pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(
+          a, s_rec, s_par, s_side, s_pgrad, ws, reinterpret_cast<const float4*>(job.ligand.beads), job.ligand.nbeads,
+          M, w, st, lane);
+      write_energrad(a, p, u, lane);
     }
   }
   if (a.stats) {
@@ -1128,7 +1134,7 @@ __global__ void fill_neighbours_kernel(GridView G, const float4* rec, int nrec, 
     const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
     if (dsq < nbdis2) nblist[s + c++] = (uint16_t)r;
   }
-  const int padded = (n + kSeg - 1) / kSeg * kSeg;
+  const int padded = (n + kListPad - 1) / kListPad * kListPad;
   for (int k = n; k < padded; ++k) nblist[s + k] = 0xFFFFu;
   nbgrid[v] = make_int2(n, (int)s);
 }
@@ -1248,9 +1254,9 @@ int device_info(DevInfo& d) {
   return ATJ_OK;
 }
 
-template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
-  auto kern = pose_kernel<GS, MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>;
+  auto kern = pose_kernel<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>;
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
@@ -1259,7 +1265,8 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
     if (per_cu < 1) per_cu = 1;
     blocks = per_cu * d.cus;
-    const long long per_block = (long long)wpb * (kWave / GS);
+    // poses a block takes per work-counter fetch round
+    const long long per_block = (long long)wpb * (MODE == kMinimize ? kBatch : 1);
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
@@ -1270,30 +1277,20 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   return ATJ_OK;
 }
 
-template <int GS, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
 int launch_layout(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
   if (a.G.layout == ATJ_LAYOUT_CORNER)
-    return launch_pose_kernel_t<GS, MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_CORNER>(a, wpb, lds, blocks, d, stream);
-  return launch_pose_kernel_t<GS, MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_PLAIN>(a, wpb, lds, blocks, d, stream);
+    return launch_pose_kernel_t<MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_CORNER>(a, wpb, lds, blocks, d, stream);
+  return launch_pose_kernel_t<MODE, REC_LDS, PLATEAU, EXACT, ATJ_LAYOUT_PLAIN>(a, wpb, lds, blocks, d, stream);
 }
 
-template <int GS, int MODE, bool REC_LDS>
+template <int MODE, bool REC_LDS>
 int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
   const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0, exact = (a.flags & ATJ_GRAD_EXACT) != 0;
-  if (plateau && exact) return launch_layout<GS, MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
-  if (plateau) return launch_layout<GS, MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
-  if (exact) return launch_layout<GS, MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
-  return launch_layout<GS, MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
-}
-
-template <int GS, int MODE>
-int launch_paths(KArgs& a, int wpb, size_t base, bool rec_global, int blocks, const DevInfo& d,
-                 hipStream_t stream) {
-  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
-  if (!rec_global && with_rec <= d.lds_per_block)
-    return launch_flags<GS, MODE, true>(a, wpb, with_rec, blocks, d, stream);
-  if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
-  return launch_flags<GS, MODE, false>(a, wpb, base, blocks, d, stream);
+  if (plateau && exact) return launch_layout<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
+  if (plateau) return launch_layout<MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
+  if (exact) return launch_layout<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
+  return launch_layout<MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
 }
 
 template <int MODE>
@@ -1306,18 +1303,19 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   const uint32_t options = launch ? launch->options : 0u;
-  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_GROUP32)) return fail(ATJ_ERR_INVALID, "unknown launch option");
-  // per-wave scratch: the same size for one 64-lane group or two 32-lane groups
-  static_assert(sizeof(GroupScratch<kWave>) == 2 * sizeof(GroupScratch<32>), "group scratch size");
+  if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
   const size_t ntp = (size_t)a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
-                      wpb * sizeof(GroupScratch<kWave>) + (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+                      wpb * sizeof(WaveScratch) + (MODE == kMinimize ? wpb * sizeof(BatchScratch) : 0) +
+                      (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  const bool rec_global = (options & ATJ_LAUNCH_REC_GLOBAL) != 0;
-  if (options & ATJ_LAUNCH_GROUP32) return launch_paths<32, MODE>(a, wpb, base, rec_global, blocks, d, stream);
-  return launch_paths<kWave, MODE>(a, wpb, base, rec_global, blocks, d, stream);
+  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
+  if (!(options & ATJ_LAUNCH_REC_GLOBAL) && with_rec <= d.lds_per_block)
+    return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
+  if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
+  return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
 }
 
 }  // namespace

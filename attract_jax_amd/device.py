@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 
-SEG = 8
+SEG = 16  # lists are padded with 0xFFFF to a multiple of SEG entries (the longest segment read)
 # Non-empty lists start on a 64-byte boundary (32 entries), so a list of up to 32 entries
 # is one memory line and a longer one the fewest lines; the head of the list array is
 # one such line of padding (read by idle lanes).
@@ -128,7 +128,7 @@ class ForceField:
 
 def lists_from_csr(counts, starts0, flat):
     """Device neighbour-list format from CSR lists (counts, 0-based starts, flat indices):
-    returns (nbgrid (V,2) int32, nblist uint16) with 0xFFFF padding to a multiple of 8
+    returns (nbgrid (V,2) int32, nblist uint16) with 0xFFFF padding to a multiple of 32
     entries, non-empty lists starting at multiples of 32 entries (>= 32) and an
     all-padding head of 32 entries."""
     counts = np.asarray(counts, np.int64).reshape(-1)
@@ -281,7 +281,7 @@ class DeviceGrid:
         self.inner_plain = self.outer_plain = None
         V = self.dim[0] * self.dim[1] * self.dim[2]
         self.nbgrid = torch.zeros((V, 2), dtype=torch.int32, device=d)
-        self.nblist = torch.zeros(SEG, dtype=torch.int16, device=d)
+        self.nblist = torch.full((LINE,), -1, dtype=torch.int16, device=d)  # replaced by the lists below
         geom = self.struct()
         st = _stream(d)
         rstruct = receptor.struct()

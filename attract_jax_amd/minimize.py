@@ -66,7 +66,7 @@ class BatchPlan:
         self.system = system
         self.ligands = list(ligands)
         self.pose_counts = [int(n) for n in pose_counts]
-        self.nsteps = int(nsteps)
+        self.nsteps = int(np.max(nsteps))  # per-job step counts may differ
         d = system.device
         self.jobs = system.jobs_tensor(self.ligands, nsteps, lr_rot, lr_trans, beta1, beta2, eps, self.pose_counts)
         self.njobs = len(self.ligands)

@@ -61,9 +61,10 @@ typedef struct AtjGrid {
   const float* inner;     /* potentials + gradients, NULL for a neighbour-only grid */
   const float* outer;     /* NULL = no outer grid */
   const int32_t* nbgrid;  /* [dx*dy*dz][2] = (count, start): 0-based start into nblist,
-                             start % 8 == 0 and start >= 8 (lists are 16-byte aligned) */
+                             start % 8 == 0 and start >= 16 (lists are 16-byte aligned; 64-byte
+                             alignment, start % 32 == 0, saves memory lines) */
   const uint16_t* nblist; /* receptor bead indices; each list padded with 0xFFFF to a multiple
-                             of 8 entries; entries 0..7 are 0xFFFF */
+                             of 16 entries; entries 0..15 are 0xFFFF */
 } AtjGrid;
 
 /* ATTRACT pair table, compacted to receptor types x ligand types
@@ -120,9 +121,6 @@ typedef struct AtjPoseState {
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
 #define ATJ_LAUNCH_REC_GLOBAL 0x1u  /* read receptor beads through the vector-memory (L1) path, not LDS */
-#define ATJ_LAUNCH_GROUP32 0x2u     /* two poses per wavefront (32 lanes each) instead of one.  Results
-                                       differ from the one-pose-per-wave form in rounding only (summation
-                                       order); a pose's result does not depend on the other poses. */
 typedef struct AtjLaunch {
   int32_t waves_per_block;  /* 1..16 (0 = default 4) */
   int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
@@ -201,8 +199,9 @@ int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff,
                               const AtjReceptor* rec, const int32_t* lig_fftype_dev,
                               float cutoff, float* inner_out, float* outer_out, void* stream);
 /* Neighbour lists: receptor beads with |p - r| < neighbourdis of every inner grid point p.
- * Pass 1 writes counts [dx*dy*dz]; the caller turns round-up-to-8 counts into 0-based
- * starts; pass 2 writes nbgrid [V][2] and nblist (ascending receptor index). */
+ * Pass 1 writes counts [dx*dy*dz]; the caller turns round-up-to-16 counts into 0-based
+ * starts (and fills entries 0..15 with 0xFFFF); pass 2 writes nbgrid [V][2] and nblist
+ * (ascending receptor index, 0xFFFF padding to a multiple of 16 entries). */
 int atj_grid_count_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis,
                               int32_t* counts, void* stream);
 int atj_grid_fill_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float neighbourdis,

@@ -306,64 +306,19 @@ def test_receptor_paths_bit_identical(s1):
     assert torch.equal(E1, E2) and torch.equal(G1, G2)
 
 
-def test_group32_parity_and_independence(s1):
-    """Two poses per wavefront: within the twin's tolerance, and a pose's result does not
-    depend on which pose shares its wavefront (bit for bit under a shifted batch); the
-    fused minimizer equals energrad + adam_step under the same launch shape."""
+def test_batch_straddles_jobs_with_different_step_counts(s1):
+    """Minimizer batches of consecutive poses that span two jobs (13 + 11 poses, 7 and 12
+    steps): every pose ends as when its job runs alone."""
     w = s1.w
-    mat = s1.mats()
-    old = s1.system.launch
-    try:
-        s1.system.launch = _lib.AtjLaunch(8, 0, _lib.ATJ_LAUNCH_GROUP32)
-        E, G = s1.system.energrad(mat, s1.ligs[0])
-        Et, Gt = s1.twins[0].energrad(mat)
-        assert _rel(E.cpu().numpy(), Et).max() < 1e-4
-        assert _grel(G.cpu().numpy(), Gt).max() < 1e-4
-        E2, G2 = s1.system.energrad(mat[1:], s1.ligs[0])
-        assert torch.equal(E[1:], E2) and torch.equal(G[1:], G2)
-        s1.system.launch = _lib.AtjLaunch(16, 0, _lib.ATJ_LAUNCH_GROUP32 | _lib.ATJ_LAUNCH_REC_GLOBAL)
-        E3, G3 = s1.system.energrad(mat, s1.ligs[0])
-        assert torch.equal(E, E3) and torch.equal(G, G3)
-        s1.system.launch = _lib.AtjLaunch(8, 0, _lib.ATJ_LAUNCH_GROUP32)
-        n = 301
-        plan = BatchPlan(s1.system, s1.ligs, [n], 25, w.lr_rot, w.lr_trans)
-        a = PoseState.create(w.quat[:n], w.trans[:n], DEV)
-        minimize(s1.system, plan, a, 25)
-        b = PoseState.create(w.quat[:n], w.trans[:n], DEV)
-        minimize_plain(s1.system, s1.ligs[0], b, 25, w.lr_rot, w.lr_trans)
-        assert torch.equal(a.pose, b.pose) and torch.equal(a.energy, b.energy)
-        Pt, Et = s1.twins[0].minimize(np.concatenate([w.quat[:n], w.trans[:n]], 1), 25, w.lr_rot, w.lr_trans)
-        P = a.pose.cpu().numpy()
-        lig = w.ligands[0]
-        xg = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), lig.coor)
-        xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
-        r = O.rmsd(xg, xt)
-        assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
-    finally:
-        s1.system.launch = old
-
-
-def test_group32_uneven_ligands():
-    """Two poses of different ligands (1, 21, 150 beads) in one wavefront."""
-    w = c3(nligands=3, poses_per_ligand=33, nsteps=12)
-    from attract_jax_amd.workloads import LigandSpec, random_chain
-    rng = np.random.default_rng(9)
-    w.ligands[0] = LigandSpec(w.ligands[0].coor[:1], w.ligands[0].types[:1])
-    w.ligands[2] = LigandSpec(random_chain(150, rng), rng.integers(1, 33, 150))
-    system, ligs = build_system(w, DEV, keep_plain=True)
-    system.launch = _lib.AtjLaunch(8, 0, _lib.ATJ_LAUNCH_GROUP32)
-    plan = BatchPlan(system, ligs, w.pose_counts, w.nsteps, w.lr_rot, w.lr_trans)
-    st = PoseState.create(w.quat, w.trans, DEV)
-    minimize(system, plan, st, w.nsteps)
-    grid = system.grid.to_reference_grid()
-    for k in range(len(ligs)):
-        sl = slice(plan.offsets[k], plan.offsets[k + 1])
-        alone_plan = BatchPlan(system, [ligs[k]], [w.pose_counts[k]], w.nsteps, w.lr_rot, w.lr_trans)
-        alone = PoseState.create(w.quat[sl], w.trans[sl], DEV)
-        minimize(system, alone_plan, alone, w.nsteps)
-        assert torch.equal(alone.pose, st.pose[sl]) and torch.equal(alone.energy, st.energy[sl])
-        l = w.ligands[k]
-        tw = Twin(grid, _ff_arrays(system), w.coor_rec, w.rec_types - 1, l.coor, l.types - 1, l.types - 1)
-        Pt, _ = tw.minimize(np.concatenate([w.quat[sl], w.trans[sl]], 1), w.nsteps, w.lr_rot, w.lr_trans)
-        d = np.abs(st.pose[sl].cpu().numpy()[:, 4:] - Pt[:, 4:]).max(axis=1)
-        assert np.median(d) < 1e-3 and (d < 0.05).mean() >= 0.9
+    ligs = [s1.ligs[0], s1.ligs[0]]
+    counts, steps = [13, 11], [7, 12]
+    q, t = w.quat[:24], w.trans[:24]
+    plan = BatchPlan(s1.system, ligs, counts, steps, w.lr_rot, w.lr_trans)
+    st = PoseState.create(q, t, DEV)
+    minimize(s1.system, plan, st, max(steps))
+    assert st.count.cpu().tolist() == [7] * 13 + [12] * 11
+    for k, (lo, hi) in enumerate([(0, 13), (13, 24)]):
+        alone_plan = BatchPlan(s1.system, [ligs[k]], [counts[k]], steps[k], w.lr_rot, w.lr_trans)
+        alone = PoseState.create(q[lo:hi], t[lo:hi], DEV)
+        minimize(s1.system, alone_plan, alone, steps[k])
+        assert torch.equal(alone.pose, st.pose[lo:hi]) and torch.equal(alone.energy, st.energy[lo:hi])
