@@ -63,17 +63,31 @@ class AtjLigand(ctypes.Structure):
     _fields_ = [("beads", ctypes.c_void_p), ("nbeads", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+ATJ_OPT_ADAM = 0
+ATJ_OPT_ADABELIEF = 1
+
+
+class AtjOptimizer(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("lr_rot", ctypes.c_float),
+        ("lr_trans", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("eps_root", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+    ]
+
+
 class AtjJob(ctypes.Structure):
     _fields_ = [
         ("ligand", AtjLigand),
         ("pose_begin", ctypes.c_int32),
         ("pose_count", ctypes.c_int32),
         ("nsteps", ctypes.c_int32),
-        ("lr_rot", ctypes.c_float),
-        ("lr_trans", ctypes.c_float),
-        ("beta1", ctypes.c_float),
-        ("beta2", ctypes.c_float),
-        ("eps", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
+        ("opt", AtjOptimizer),
     ]
 
 
@@ -111,7 +125,8 @@ SIGNATURES = {
     "atj_minimize": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjForceField), ctypes.POINTER(AtjReceptor),
                       _P, _I, _P, _P, _I, ctypes.POINTER(AtjPoseState), _I, _U, ctypes.POINTER(AtjLaunch),
                       _P, _P, _P], ctypes.c_int),
-    "atj_adam_step": ([_P, _I, ctypes.POINTER(AtjPoseState), _F, _F, _F, _F, _F, _P], ctypes.c_int),
+    "atj_optimizer_step": ([_P, _P, _I, ctypes.POINTER(AtjPoseState), ctypes.POINTER(AtjOptimizer), _P],
+                           ctypes.c_int),
     "atj_energrad_adjoint": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjForceField),
                               ctypes.POINTER(AtjReceptor), _P, _I, _P, _P, _I, _P, _P, _P, _P, _U,
                               ctypes.POINTER(AtjLaunch), _P, _P], ctypes.c_int),

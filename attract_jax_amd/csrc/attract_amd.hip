@@ -686,11 +686,16 @@ __device__ __forceinline__ void write_energrad(const KArgs& a, int p, float u, i
   else o[4 * (i - 13) + 3] = 0.f;  // i = 13..15: the zero column of rows 0..2
 }
 
-// One ADAM step of a minimizer slot (optax.adam: bias-corrected moments, eps outside the
-// sqrt, then -lr; test4.py:445-455) followed by Rotation.from_quat(as_quat())
-// renormalisation, with the same operations in the same order as adam_step_kernel.
-__device__ __forceinline__ void adam_update(float prm[7], float m1[7], float m2[7], const float acc[kAcc], float c1,
-                                            float c2, float lr_rot, float lr_trans, float b1, float b2, float eps) {
+// One optimizer step of a pose from its evaluation (acc: energy, dE/dM) with the same
+// operations in the same order in the fused minimizer and in optimizer_step_kernel:
+// quaternion chain rule through as_matrix, optional gravity restraint w |t|^2
+// (test5.py:418-420; energy += (|t| |t|) w, translation gradient += 2 w t), then the
+// optax update (adam, test4.py:447-453, or adabelief, test5.py:451-457: the belief is
+// the moment of g - m_prev, plus eps_root), bias-corrected by c1 = 1 - b1^t and
+// c2 = 1 - b2^t, -lr, and Rotation.from_quat(as_quat()) renormalisation.  Returns the
+// pose's energy including the restraint.
+__device__ __forceinline__ float optimizer_update(const AtjOptimizer& o, float prm[7], float m1[7], float m2[7],
+                                                  const float acc[kAcc], float c1, float c2) {
   const float q[4] = {prm[0], prm[1], prm[2], prm[3]};
   const float A[9] = {acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7], acc[8], acc[9]};
   float gr[7];
@@ -698,16 +703,28 @@ __device__ __forceinline__ void adam_update(float prm[7], float m1[7], float m2[
   gr[4] = acc[10];
   gr[5] = acc[11];
   gr[6] = acc[12];
+  float E = acc[0];
+  if (o.gravity != 0.f) {
+    const float n = sqrtf(fmaf(prm[6], prm[6], fmaf(prm[5], prm[5], prm[4] * prm[4])));
+    E = E + (n * n) * o.gravity;
+#pragma unroll
+    for (int i = 4; i < 7; ++i) gr[i] = gr[i] + (2.f * o.gravity) * prm[i];
+  }
+  const float b1 = o.beta1, b2 = o.beta2;
+  const bool belief = o.kind == ATJ_OPT_ADABELIEF;
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
+    const float pe = belief ? gr[i] - m1[i] : gr[i];
     m1[i] = (1.f - b1) * gr[i] + b1 * m1[i];
-    m2[i] = (1.f - b2) * (gr[i] * gr[i]) + b2 * m2[i];
-    const float upd = (m1[i] / c1) / (sqrtf(m2[i] / c2) + eps);
-    prm[i] = prm[i] + (-(i < 4 ? lr_rot : lr_trans)) * upd;
+    m2[i] = (1.f - b2) * (pe * pe) + b2 * m2[i];
+    if (belief) m2[i] = m2[i] + o.eps_root;
+    const float upd = (m1[i] / c1) / (sqrtf(m2[i] / c2) + o.eps);
+    prm[i] = prm[i] + (-(i < 4 ? o.lr_rot : o.lr_trans)) * upd;
   }
   const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
 #pragma unroll
   for (int i = 0; i < 4; ++i) prm[i] = prm[i] / nrm;
+  return E;
 }
 
 // Rotation rows and translation (12 floats) of a (quaternion, translation) pose.
@@ -792,8 +809,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
       nst = max(0, min(a.nsteps_launch, job.nsteps - cnt));
       // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
       for (int i = 0; i < cnt; ++i) {
-        c1 = fmaf(job.beta1, c1, 1.f - job.beta1);
-        c2 = fmaf(job.beta2, c2, 1.f - job.beta2);
+        c1 = fmaf(job.opt.beta1, c1, 1.f - job.opt.beta1);
+        c2 = fmaf(job.opt.beta2, c2, 1.f - job.opt.beta2);
       }
       float prm[7], M[12];
 #pragma unroll
@@ -836,11 +853,10 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
         }
 #pragma unroll
         for (int i = 0; i < kAcc; ++i) acc[i] = bs->acc[lane][i];
-        elast = acc[0];
         ++cnt;
-        c1 = fmaf(job.beta1, c1, 1.f - job.beta1);
-        c2 = fmaf(job.beta2, c2, 1.f - job.beta2);
-        adam_update(prm, m1, m2, acc, c1, c2, job.lr_rot, job.lr_trans, job.beta1, job.beta2, job.eps);
+        c1 = fmaf(job.opt.beta1, c1, 1.f - job.opt.beta1);
+        c2 = fmaf(job.opt.beta2, c2, 1.f - job.opt.beta2);
+        elast = optimizer_update(job.opt, prm, m1, m2, acc, c1, c2);
 #pragma unroll
         for (int i = 0; i < 7; ++i) {
           bs->prm[lane][i] = prm[i];
@@ -994,42 +1010,36 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
 }
 
 // Plain one-launch-per-step ADAM (the per-job comparison form).
-__global__ void adam_step_kernel(const float* dEdM, int N, float* pose7, float* mom1, float* mom2,
-                                 int* count, float lr_rot, float lr_trans, float b1, float b2, float eps) {
+__global__ void optimizer_step_kernel(const float* dEdM, float* energy, int N, float* pose7, float* mom1,
+                                      float* mom2, int* count, AtjOptimizer o) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const float* G = dEdM + (size_t)p * 16;
   float* P = pose7 + (size_t)p * 7;
-  float q[4] = {P[0], P[1], P[2], P[3]};
-  const float A[9] = {G[0], G[1], G[2], G[4], G[5], G[6], G[8], G[9], G[10]};
-  float gr[7];
-  quat_grad(q, A, gr);
-  gr[4] = G[12];
-  gr[5] = G[13];
-  gr[6] = G[14];
+  const float acc[kAcc] = {energy ? energy[p] : 0.f, G[0], G[1], G[2], G[4], G[5], G[6], G[8], G[9], G[10],
+                           G[12], G[13], G[14]};
   const int cnt = count[p] + 1;
   // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
-  float bc1 = 0.f, bc2 = 0.f;
+  float c1 = 0.f, c2 = 0.f;
   for (int i = 0; i < cnt; ++i) {
-    bc1 = fmaf(b1, bc1, 1.f - b1);
-    bc2 = fmaf(b2, bc2, 1.f - b2);
+    c1 = fmaf(o.beta1, c1, 1.f - o.beta1);
+    c2 = fmaf(o.beta2, c2, 1.f - o.beta2);
   }
-  float prm[7] = {q[0], q[1], q[2], q[3], P[4], P[5], P[6]};
+  float prm[7], m1[7], m2[7];
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
-    float m = mom1[7 * p + i], v = mom2[7 * p + i];
-    m = (1.f - b1) * gr[i] + b1 * m;
-    v = (1.f - b2) * (gr[i] * gr[i]) + b2 * v;
-    mom1[7 * p + i] = m;
-    mom2[7 * p + i] = v;
-    const float upd = (m / bc1) / (sqrtf(v / bc2) + eps);
-    prm[i] = prm[i] + (-(i < 4 ? lr_rot : lr_trans)) * upd;
+    prm[i] = P[i];
+    m1[i] = mom1[7 * p + i];
+    m2[i] = mom2[7 * p + i];
   }
-  const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+  const float E = optimizer_update(o, prm, m1, m2, acc, c1, c2);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) P[i] = prm[i] / nrm;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) P[4 + i] = prm[4 + i];
+  for (int i = 0; i < 7; ++i) {
+    P[i] = prm[i];
+    mom1[7 * p + i] = m1[i];
+    mom2[7 * p + i] = m2[i];
+  }
+  if (energy) energy[p] = E;
   count[p] = cnt;
 }
 
@@ -1438,14 +1448,14 @@ int atj_point_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjRecep
   return ATJ_OK;
 }
 
-int atj_adam_step(const float* dEdM, int32_t N, const AtjPoseState* state, float lr_rot, float lr_trans,
-                  float beta1, float beta2, float eps, void* stream) {
+int atj_optimizer_step(const float* dEdM, float* energy, int32_t N, const AtjPoseState* state,
+                       const AtjOptimizer* opt, void* stream) {
   if (N <= 0) return N == 0 ? ATJ_OK : fail(ATJ_ERR_INVALID, "N < 0");
-  if (!dEdM || !state || !state->pose || !state->m || !state->v || !state->count)
+  if (!dEdM || !state || !opt || !state->pose || !state->m || !state->v || !state->count)
     return fail(ATJ_ERR_INVALID, "NULL argument");
-  hipLaunchKernelGGL(adam_step_kernel, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, dEdM, N,
-                     state->pose, state->m, state->v, state->count, lr_rot, lr_trans, beta1, beta2,
-                     eps);
+  if (opt->kind != ATJ_OPT_ADAM && opt->kind != ATJ_OPT_ADABELIEF) return fail(ATJ_ERR_INVALID, "unknown optimizer");
+  hipLaunchKernelGGL(optimizer_step_kernel, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, dEdM, energy,
+                     N, state->pose, state->m, state->v, state->count, *opt);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }

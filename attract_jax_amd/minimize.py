@@ -1,16 +1,20 @@
-"""Batched rigid-body ADAM minimization (test4.py:430-455) on the device.
+"""Batched rigid-body minimization (test4.py:430-455, test5.py:446-458) on the device.
 
 `minimize` packs the poses of any number of ligands (uneven bead counts) into ONE
-persistent launch that runs every step with the pose state in registers
-(atj_minimize).  `minimize_plain` is the per-job comparison form: one ligand at a
-time, one energrad launch plus one ADAM launch per step.
+persistent launch that runs every step (atj_minimize).  `minimize_plain` is the
+per-job comparison form: one ligand at a time, one energrad launch plus one
+optimizer launch per step.
 
-Optimizer semantics (optax.adam defaults, test4.py:446-453): per pose and per
-parameter, m = (1-b1) g + b1 m, v = (1-b2) g^2 + b2 v, update = -lr m_hat /
-(sqrt(v_hat) + eps) with bias-corrected moments; the quaternion is renormalised
-after every step (Rotation.from_quat(as_quat()), test4.py:454).  The reference uses
-one learning rate; lr_rot / lr_trans may differ.  The energy reported per pose is
-the one evaluated in the last step, before its update (test4.py:450,455).
+Optimizer semantics, per pose and per parameter (quaternion x, y, z, w and
+translation): optax.adam defaults (test4.py:446-453): m = (1-b1) g + b1 m,
+v = (1-b2) g^2 + b2 v; optax.adabelief (test5.py:451-457): the second moment is
+taken of g - m_prev and eps_root is added to it; both update -lr m_hat /
+(sqrt(v_hat) + eps) with bias-corrected moments, and the quaternion is renormalised
+after every step (Rotation.from_quat(as_quat())).  gravity > 0 adds gravity |t|^2 to
+the energy (test5.py:418-420).  The reference uses one learning rate; lr_rot /
+lr_trans may differ.  The energy reported per pose is the one evaluated in the last
+step, before its update (test4.py:450,455).  Zeroing count (and the moments)
+re-initialises the optimizer, as test5.py:452 does at every stage.
 """
 import ctypes
 from dataclasses import dataclass
@@ -62,13 +66,13 @@ class BatchPlan:
     persistent launch is made of short poses)."""
 
     def __init__(self, system, ligands, pose_counts, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999,
-                 eps=1e-8):
+                 eps=1e-8, *, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
         self.system = system
         self.ligands = list(ligands)
         self.pose_counts = [int(n) for n in pose_counts]
-        self.nsteps = int(np.max(nsteps))  # per-job step counts may differ
         d = system.device
-        self.jobs = system.jobs_tensor(self.ligands, nsteps, lr_rot, lr_trans, beta1, beta2, eps, self.pose_counts)
+        self.set_optimizer(nsteps, lr_rot, lr_trans, beta1, beta2, eps, optimizer=optimizer, eps_root=eps_root,
+                           gravity=gravity)
         self.njobs = len(self.ligands)
         job_of = np.repeat(np.arange(self.njobs, dtype=np.int32), self.pose_counts)
         self.pose_job = torch.from_numpy(job_of).to(d)
@@ -77,6 +81,14 @@ class BatchPlan:
         self.order = torch.from_numpy(order).to(d)
         self.N = int(sum(self.pose_counts))
         self.offsets = np.concatenate(([0], np.cumsum(self.pose_counts)))
+
+    def set_optimizer(self, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8, *,
+                      optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
+        """(Re)write the job table: step counts and optimizer settings (per job or shared)."""
+        self.nsteps = int(np.max(nsteps))  # per-job step counts may differ
+        self.jobs = self.system.jobs_tensor(self.ligands, nsteps, lr_rot, lr_trans, beta1, beta2, eps,
+                                            self.pose_counts, optimizer=optimizer, eps_root=eps_root,
+                                            gravity=gravity)
 
 
 def minimize(system, plan, state, nsteps_launch, flags=None, stats=False, stream=None):
@@ -92,9 +104,9 @@ def minimize(system, plan, state, nsteps_launch, flags=None, stats=False, stream
 
 
 def minimize_plain(system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8,
-                   flags=None, stream=None, jobs=None):
+                   flags=None, stream=None, jobs=None, *, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
     """Per-job form: for each step, one energrad launch (quaternion poses) then one
-    ADAM launch; pose state lives in HBM between launches."""
+    optimizer launch; pose state lives in HBM between launches."""
     lib = _lib.load()
     d = system.device
     st = stream if stream is not None else _stream(d)
@@ -104,11 +116,12 @@ def minimize_plain(system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, b
     dEdM = torch.empty((N, 16), dtype=torch.float32, device=d)
     fl = ((system.flags if flags is None else flags) | _lib.ATJ_POSE_QT)
     ss = state.struct()
+    opt = _lib.AtjOptimizer(int(optimizer), float(lr_rot), float(lr_trans), float(beta1), float(beta2), float(eps),
+                            float(eps_root), float(gravity))
     for _ in range(int(nsteps)):
         _lib.check(lib.atj_energrad(ctypes.byref(system._gs), ctypes.byref(system._fs), ctypes.byref(system._rs),
                                     _ptr(jobs), 1, ctypes.c_void_p(0), ctypes.c_void_p(0), _ptr(state.pose), N,
                                     _ptr(state.energy), _ptr(dEdM), fl, ctypes.byref(system.launch),
                                     _ptr(system.counter), ctypes.c_void_p(0), st))
-        _lib.check(lib.atj_adam_step(_ptr(dEdM), N, ctypes.byref(ss), float(lr_rot), float(lr_trans),
-                                     float(beta1), float(beta2), float(eps), st))
+        _lib.check(lib.atj_optimizer_step(_ptr(dEdM), _ptr(state.energy), N, ctypes.byref(ss), ctypes.byref(opt), st))
     return state

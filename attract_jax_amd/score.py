@@ -138,18 +138,20 @@ class ScoringSystem:
         self._rs = self.receptor.struct()
 
     def jobs_tensor(self, ligands, nsteps=0, lr_rot=1e-3, lr_trans=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
-                    pose_counts=None):
-        """Device array of AtjJob records for the given ligands."""
+                    pose_counts=None, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
+        """Device array of AtjJob records for the given ligands (per-ligand arrays or
+        scalars for nsteps, lr_rot, lr_trans, gravity)."""
         if not isinstance(ligands, (list, tuple)):
             ligands = [ligands]
         arr = (_lib.AtjJob * len(ligands))()
         begin = 0
+        pick = lambda v, i: float(v[i]) if np.ndim(v) else float(v)
         for i, lig in enumerate(ligands):
             n = int(pose_counts[i]) if pose_counts is not None else 0
             ns = int(nsteps[i]) if np.ndim(nsteps) else int(nsteps)
-            lrr = float(lr_rot[i]) if np.ndim(lr_rot) else float(lr_rot)
-            lrt = float(lr_trans[i]) if np.ndim(lr_trans) else float(lr_trans)
-            arr[i] = _lib.AtjJob(lig.struct(), begin, n, ns, lrr, lrt, beta1, beta2, eps)
+            opt = _lib.AtjOptimizer(int(optimizer), pick(lr_rot, i), pick(lr_trans, i), float(beta1), float(beta2),
+                                    float(eps), float(eps_root), pick(gravity, i))
+            arr[i] = _lib.AtjJob(lig.struct(), begin, n, ns, 0, opt)
             begin += n
         raw = np.frombuffer(bytes(arr), np.uint8)
         return torch.from_numpy(raw.copy()).to(self.device)

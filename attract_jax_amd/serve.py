@@ -324,8 +324,8 @@ class GpuBackend:
         from . import _lib
         arr = (_lib.AtjJob * len(jobs))()
         for i, j in enumerate(jobs):
-            arr[i] = _lib.AtjJob(ligs[i].struct(), j.slot, j.npose, j.spec.nsteps, j.spec.lr_rot, j.spec.lr_trans,
-                                 0.9, 0.999, 1e-8)
+            opt = _lib.AtjOptimizer(_lib.ATJ_OPT_ADAM, j.spec.lr_rot, j.spec.lr_trans, 0.9, 0.999, 1e-8, 0.0, 0.0)
+            arr[i] = _lib.AtjJob(ligs[i].struct(), j.slot, j.npose, j.spec.nsteps, 0, opt)
         jobs_t = torch.from_numpy(np.frombuffer(bytes(arr), np.uint8).copy()).to(self.device)
         cap = g.handle["state"].N
         pose_job = np.zeros(cap, np.int32)

@@ -94,15 +94,33 @@ typedef struct AtjLigand {
   int32_t reserved;
 } AtjLigand;
 
+/* Optimizer of the rigid-body minimization (per job).  Both update the parameter pytree
+ * (quaternion (x, y, z, w), translation) like optax, then renormalise the quaternion
+ * (Rotation.from_quat(as_quat())).
+ *   ATJ_OPT_ADAM       optax.adam (test4.py:447-453): m = (1-b1) g + b1 m, v = (1-b2) g^2 + b2 v
+ *   ATJ_OPT_ADABELIEF  optax.adabelief (test5.py:451-457): s = (1-b2) (g - m_prev)^2 + b2 s + eps_root
+ * with the update -lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps).  gravity > 0 adds
+ * gravity * |t|^2 to each pose's energy and 2 gravity t to its translation gradient
+ * (test5.py:418-420, weight 0.1 in the stages that use it). */
+#define ATJ_OPT_ADAM 0
+#define ATJ_OPT_ADABELIEF 1
+typedef struct AtjOptimizer {
+  int32_t kind;     /* ATJ_OPT_* */
+  float lr_rot;     /* learning rate of the quaternion */
+  float lr_trans;   /* learning rate of the translation (the reference uses one lr for both) */
+  float beta1, beta2, eps;
+  float eps_root;   /* adabelief only (optax default 1e-16) */
+  float gravity;    /* restraint weight, 0 = off */
+} AtjOptimizer;
+
 /* A group of poses sharing one ligand (one docking job, or one ligand of a batch). */
 typedef struct AtjJob {
   AtjLigand ligand;
   int32_t pose_begin;  /* first pose slot of the job */
   int32_t pose_count;
-  int32_t nsteps;      /* total ADAM steps of the job (minimize only) */
-  float lr_rot;        /* ADAM learning rate of the quaternion */
-  float lr_trans;      /* ADAM learning rate of the translation */
-  float beta1, beta2, eps;
+  int32_t nsteps;      /* total optimizer steps of the job (minimize only) */
+  int32_t reserved;
+  AtjOptimizer opt;    /* minimize only */
 } AtjJob;
 
 /* Optional per-launch counters (device uint64[4]): beads inside a grid,
@@ -113,10 +131,10 @@ typedef struct AtjJob {
 /* Pose-state arrays of the minimizer (all device, indexed by pose slot). */
 typedef struct AtjPoseState {
   float* pose;      /* [N][7] = quaternion (x, y, z, w), translation (x, y, z) */
-  float* m;         /* [N][7] ADAM first moment (quat, trans) */
-  float* v;         /* [N][7] ADAM second moment */
-  int32_t* count;   /* [N] ADAM steps done */
-  float* energy;    /* [N] energy of the last evaluated pose */
+  float* m;         /* [N][7] first moment (quat, trans) */
+  float* v;         /* [N][7] second moment (adam) / belief (adabelief) */
+  int32_t* count;   /* [N] optimizer steps done (zero them to re-initialise the optimizer) */
+  float* energy;    /* [N] energy of the last evaluated pose (gravity term included) */
 } AtjPoseState;
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
@@ -161,21 +179,23 @@ int atj_point_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjRecep
                     const int32_t* types, int32_t M, float* grid4, int32_t* voxel, float* nbene, float* nbgrad,
                     uint32_t flags, void* stream);
 
-/* Batched rigid-body ADAM minimization: every pose of every job runs
- * min(nsteps_launch, job.nsteps - count) steps inside ONE persistent launch;
- * pose state stays in registers between steps (reference loop: test4.py:445-455,
- * optax.adam defaults, quaternion renormalised after every step).
- * state.energy receives the energy evaluated in the last step. */
+/* Batched rigid-body minimization: every pose of every job runs
+ * min(nsteps_launch, job.nsteps - count) steps of its job's optimizer inside ONE
+ * persistent launch (reference loops: test4.py:445-455 adam, test5.py:450-458 adabelief
+ * + gravity; quaternion renormalised after every step).  state.energy receives the
+ * energy evaluated in the last step. */
 int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
                  const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
                  const int32_t* pose_order, int32_t N, const AtjPoseState* state,
                  int32_t nsteps_launch, uint32_t flags, const AtjLaunch* launch,
                  int32_t* counter, unsigned long long* stats, void* stream);
 
-/* One ADAM step of N poses from dE/dM (the plain one-launch-per-step form):
- * quaternion chain rule through as_matrix, optax adam update, renormalisation. */
-int atj_adam_step(const float* dEdM, int32_t N, const AtjPoseState* state, float lr_rot,
-                  float lr_trans, float beta1, float beta2, float eps, void* stream);
+/* One optimizer step of N poses from dE/dM (the plain one-launch-per-step form):
+ * quaternion chain rule through as_matrix, the gravity term (energy[p] += gravity |t|^2
+ * when energy is not NULL), the optimizer update, renormalisation.  Same arithmetic as
+ * one step of atj_minimize. */
+int atj_optimizer_step(const float* dEdM, float* energy, int32_t N, const AtjPoseState* state,
+                       const AtjOptimizer* opt, void* stream);
 
 /* Adjoint of the pair term w.r.t. the pair table: for weights w[N]
  * (upstream dL/dE_pose) accumulates sum_pose w * dE_pose/d(rc, ac, emin) into

@@ -21,7 +21,10 @@ PINS = {
     "grid_energrad": "same as trilinear + gradient sign convention (-interpolated stored gradient)",
     "nb_energrad": "brute-force all-pairs within plateaudis on tiny systems; force-shift closed form",
     "energrad": "dE/dM = sum_j xh_j (x) g_j closed form; exact mode vs FP64 central differences",
-    "adam": "textbook Kingma & Ba with optax defaults (bias-corrected, eps outside sqrt)",
+    "optax_minimize": "adam: textbook Kingma & Ba steps and the constant-gradient property "
+                      "(each step moves -lr g/(|g|+eps)); adabelief: constant-gradient closed form of "
+                      "the belief recurrence; gravity: first-step closed form.  Against optax itself "
+                      "parity unpinned (optax absent, version unpinned)",
     "rmsd": "closed form",
     "read_grid": "golden vectors from the reference reader on a synthetic grid file",
     "build_grid": "brute force over receptor beads at sampled grid points",
@@ -299,11 +302,18 @@ def dof_gradient(mat, dEdM):
 
 # --------------------------------------------------------------------------- minimization
 
-def adam_minimize(q, t, nsteps, lr_rot, lr_trans, energy_fn, beta1=0.9, beta2=0.999, eps=1e-8):
-    """optax.adam loop of test4.py:445-455 on (quaternion, translation):
-    m = (1-b1) g + b1 m; v = (1-b2) g^2 + b2 v; p += -lr (m/(1-b1^k)) / (sqrt(v/(1-b2^k)) + eps);
-    q renormalised after each step.  energy_fn(q, t) -> (E (N,), dE/dM (N,4,4)).
-    Returns final q, t and the energies of the last evaluation."""
+def optax_minimize(q, t, nsteps, lr_rot, lr_trans, energy_fn, kind="adam", beta1=0.9, beta2=0.999, eps=1e-8,
+                   eps_root=0.0, gravity=0.0):
+    """Optimizer loops of test4.py:445-455 (optax.adam) and test5.py:450-458 (optax.adabelief)
+    on (quaternion, translation), the published optax algorithms (version unpinned):
+      adam:      m = (1-b1) g + b1 m;  v = (1-b2) g^2 + b2 v
+      adabelief: m = (1-b1) g + b1 m;  s = (1-b2) (g - m_prev)^2 + b2 s + eps_root
+                 (optax's scale_by_belief takes the prediction error against the previous
+                 first moment and adds eps_root to the stored second moment)
+      p += -lr (m / (1 - b1^k)) / (sqrt(v / (1 - b2^k)) + eps); q renormalised after each step.
+    gravity > 0: energy += gravity |t|^2, dE/dt += 2 gravity t (test5.py:418-420).
+    energy_fn(q, t) -> (E (N,), dE/dM (N,4,4)).  Returns final q, t and the energies of
+    the last evaluation (restraint included)."""
     q = np.array(q, np.float64)
     t = np.array(t, np.float64)
     N = len(q)
@@ -314,14 +324,25 @@ def adam_minimize(q, t, nsteps, lr_rot, lr_trans, energy_fn, beta1=0.9, beta2=0.
     for k in range(1, nsteps + 1):
         E, G = energy_fn(q, t)
         g = np.concatenate([quat_grad(q, G[:, :3, :3]), G[:, 3, :3]], axis=1)
+        if gravity:
+            E = E + gravity * (t * t).sum(axis=1)
+            g[:, 4:] += 2 * gravity * t
+        pe = g - m if kind == "adabelief" else g
         m = (1 - beta1) * g + beta1 * m
-        v = (1 - beta2) * g * g + beta2 * v
+        v = (1 - beta2) * pe * pe + beta2 * v
+        if kind == "adabelief":
+            v = v + eps_root
         mh = m / (1 - beta1 ** k)
         vh = v / (1 - beta2 ** k)
         p = np.concatenate([q, t], axis=1) - lr * mh / (np.sqrt(vh) + eps)
         q = p[:, :4] / np.linalg.norm(p[:, :4], axis=1, keepdims=True)
         t = p[:, 4:]
     return q, t, E
+
+
+def adam_minimize(q, t, nsteps, lr_rot, lr_trans, energy_fn, beta1=0.9, beta2=0.999, eps=1e-8):
+    """optax.adam loop of test4.py:445-455 (optax_minimize with kind="adam")."""
+    return optax_minimize(q, t, nsteps, lr_rot, lr_trans, energy_fn, "adam", beta1, beta2, eps)
 
 
 def rmsd(xa, xb):

@@ -282,10 +282,14 @@ int twin_bead_terms(const TwinGrid* G, const TwinFF* ff, const float* rec4, cons
   return 0;
 }
 
-/* ADAM minimization; pose7 [N][7] updated in place, energy = last evaluated energy */
+/* Rigid-body minimization; pose7 [N][7] updated in place, energy = last evaluated energy
+ * (gravity term included).  kind 0: optax.adam (test4.py:447-453); kind 1: optax.adabelief
+ * (test5.py:451-457; second moment of g - m_prev, plus eps_root); gravity adds
+ * gravity |t|^2 to the energy (test5.py:418-420).  Bias corrections by powf as optax
+ * computes 1 - b**count. */
 int twin_minimize(const TwinGrid* G, const TwinFF* ff, const float* rec4, const float* lig4, int nb, float* pose7,
-                  int N, int nsteps, float lr_rot, float lr_trans, float b1, float b2, float eps, unsigned flags,
-                  float* energy, int nthreads) {
+                  int N, int nsteps, float lr_rot, float lr_trans, float b1, float b2, float eps, int kind,
+                  float eps_root, float gravity, unsigned flags, float* energy, int nthreads) {
   set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int p = 0; p < N; ++p) {
@@ -308,10 +312,17 @@ int twin_minimize(const TwinGrid* G, const TwinFF* ff, const float* rec4, const 
       gr[4] = acc[10];
       gr[5] = acc[11];
       gr[6] = acc[12];
+      if (gravity != 0.f) {
+        const float n = sqrtf(fmaf(P[6], P[6], fmaf(P[5], P[5], P[4] * P[4])));
+        e = e + (n * n) * gravity;
+        for (int i = 4; i < 7; ++i) gr[i] = gr[i] + (2.f * gravity) * P[i];
+      }
       const float bc1 = 1.f - powf(b1, (float)k), bc2 = 1.f - powf(b2, (float)k);
       for (int i = 0; i < 7; ++i) {
+        const float pe = kind == 1 ? gr[i] - m[i] : gr[i];
         m[i] = (1.f - b1) * gr[i] + b1 * m[i];
-        v[i] = (1.f - b2) * (gr[i] * gr[i]) + b2 * v[i];
+        v[i] = (1.f - b2) * (pe * pe) + b2 * v[i];
+        if (kind == 1) v[i] = v[i] + eps_root;
         const float upd = (m[i] / bc1) / (sqrtf(v[i] / bc2) + eps);
         P[i] = P[i] + (-(i < 4 ? lr_rot : lr_trans)) * upd;
       }

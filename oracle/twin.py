@@ -115,7 +115,9 @@ class Twin:
                             _p(out["xp"]), ctypes.c_int(nthreads))
         return out
 
-    def minimize(self, pose7, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8, flags=None, nthreads=0):
+    def minimize(self, pose7, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8, flags=None, nthreads=0,
+                 kind=0, eps_root=0.0, gravity=0.0):
+        """kind 0 = optax.adam, 1 = optax.adabelief; gravity = restraint weight."""
         lib = load()
         P = np.array(pose7, np.float32, order="C").reshape(-1, 7)
         N = len(P)
@@ -124,6 +126,7 @@ class Twin:
         lib.twin_minimize(ctypes.byref(self.grid), ctypes.byref(self.ff), _p(self.rec4), _p(self.lig4),
                           ctypes.c_int(self.nb), _p(P), ctypes.c_int(N), ctypes.c_int(int(nsteps)),
                           ctypes.c_float(lr_rot), ctypes.c_float(lr_trans), ctypes.c_float(beta1),
-                          ctypes.c_float(beta2), ctypes.c_float(eps), ctypes.c_uint(fl), _p(E),
+                          ctypes.c_float(beta2), ctypes.c_float(eps), ctypes.c_int(int(kind)),
+                          ctypes.c_float(eps_root), ctypes.c_float(gravity), ctypes.c_uint(fl), _p(E),
                           ctypes.c_int(nthreads))
         return P, E

@@ -230,6 +230,59 @@ def test_adam_textbook_steps():
     assert np.allclose(t[0], exp_t - lr * upd[0, 4:], rtol=1e-10, atol=1e-14)
 
 
+def _translation_gradient_fn(g_t, energy=0.0):
+    """Energy function with a constant translation gradient and no rotational gradient."""
+    def fn(q, t):
+        G = np.zeros((len(q), 4, 4))
+        G[:, 3, :3] = g_t
+        return np.full(len(q), energy), G
+    return fn
+
+
+def test_adabelief_constant_gradient_closed_form():
+    """optax.adabelief under a constant gradient g: m_hat = g exactly, the prediction error
+    of step k is g b1^(k-1) (taken against the previous first moment), so
+    s_k = sum_j b2^(k-j) ((1-b2) g^2 b1^(2(j-1)) + eps_root) and each step moves t by
+    -lr g / (sqrt(s_k / (1-b2^k)) + eps).  The quaternion has zero gradient and stays put."""
+    g = np.array([0.7, -2.0, 0.05])
+    b1, b2, eps, eps_root, lr, K = 0.9, 0.999, 1e-16, 1e-16, 0.5, 12
+    q0 = np.array([[0.0, 0.0, 0.0, 1.0]])
+    t0 = np.array([[1.0, 2.0, 3.0]])
+    q, t, _ = O.optax_minimize(q0, t0, K, lr, lr, _translation_gradient_fn(g), "adabelief", b1, b2, eps, eps_root)
+    exp = t0[0].copy()
+    for k in range(1, K + 1):
+        j = np.arange(1, k + 1)
+        s = (b2 ** (k - j) * ((1 - b2) * g[:, None] ** 2 * b1 ** (2 * (j - 1)) + eps_root)).sum(axis=1)
+        exp = exp - lr * g / (np.sqrt(s / (1 - b2 ** k)) + eps)
+    assert np.allclose(t[0], exp, rtol=1e-12, atol=1e-12)
+    assert np.allclose(q, q0)
+
+
+def test_adam_constant_gradient_moves_lr_per_step():
+    """optax.adam under a constant gradient: m_hat = g and v_hat = g^2 exactly, so every step
+    moves each translation component by -lr g / (|g| + eps)."""
+    g = np.array([0.7, -2.0, 0.05])
+    q0 = np.array([[0.0, 0.0, 0.0, 1.0]])
+    t0 = np.array([[1.0, 2.0, 3.0]])
+    q, t, _ = O.optax_minimize(q0, t0, 9, 0.1, 0.1, _translation_gradient_fn(g), "adam")
+    assert np.allclose(t[0], t0[0] - 9 * 0.1 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["adam", "adabelief"])
+def test_gravity_first_step(kind):
+    """Gravity restraint w |t|^2 (test5.py:418-420): the first evaluation reports w |t0|^2 and
+    the first step sees the gradient 2 w t0."""
+    w, lr = 0.1, 0.5
+    q0 = np.array([[0.0, 0.0, 0.0, 1.0]])
+    t0 = np.array([[3.0, -4.0, 12.0]])
+    q, t, E = O.optax_minimize(q0, t0, 1, lr, lr, _translation_gradient_fn(np.zeros(3), energy=2.5), kind,
+                               eps_root=1e-16 if kind == "adabelief" else 0.0, gravity=w)
+    assert np.isclose(E[0], 2.5 + w * 169.0)
+    g = 2 * w * t0[0]
+    vh = g * g + (1e-16 / (1 - 0.999) if kind == "adabelief" else 0.0)
+    assert np.allclose(t[0], t0[0] - lr * g / (np.sqrt(vh) + 1e-8), rtol=1e-12)
+
+
 def test_rmsd_closed_form():
     rng = np.random.default_rng(12)
     x = rng.normal(size=(3, 9, 3))

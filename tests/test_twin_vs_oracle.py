@@ -69,3 +69,23 @@ def test_twin_minimize_vs_oracle(tiny):
     r = O.rmsd(x_o, x_t)
     assert np.median(r) < 0.01
     assert (r < 0.05).mean() >= 0.9
+
+
+def test_twin_adabelief_gravity_vs_oracle(tiny):
+    """test5.py's optimizer: adabelief with the gravity restraint, twin against oracle."""
+    s, tw = tiny
+    q0, t0 = s.quat[:40], s.trans[:40]
+    lr, nst, w = 0.05, 20, 0.1
+
+    def fn(q, t):
+        return s.oracle_energrad(O.mat_from_qt(q, t))
+
+    q, t, E = O.optax_minimize(q0, t0, nst, lr, lr, fn, "adabelief", eps=1e-16, eps_root=1e-16, gravity=w)
+    P, Et = tw.minimize(np.concatenate([q0, t0], 1), nst, lr, lr, eps=1e-16, kind=1, eps_root=1e-16, gravity=w)
+    x_o = O.transform(O.mat_from_qt(q, t), s.coor_lig)
+    x_t = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), s.coor_lig)
+    r = O.rmsd(x_o, x_t)
+    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.9
+    close = r < 1e-4
+    assert close.mean() >= 0.5
+    assert (np.abs(Et[close] - E[close]) / np.maximum(np.abs(E[close]), 1.0)).max() < 1e-3
