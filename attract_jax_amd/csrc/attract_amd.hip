@@ -940,6 +940,27 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   }
 }
 
+// Ligand RMSD of poses (test-rmsd.py:13-17): one wavefront per pose, lanes over beads,
+// sum of squared deviations reduced across the wave in a fixed order.
+__global__ void ligand_rmsd_kernel(const float* __restrict__ poses, int N, const float* __restrict__ lig,
+                                   const float* __restrict__ ref, int nb, float* __restrict__ out) {
+  const int lane = threadIdx.x % kWave;
+  const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (p >= N) return;  // whole waves exit together
+  float M[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) M[i] = uni(poses[(size_t)p * 16 + i]);
+  float acc = 0.f;
+  for (int j = lane; j < nb; j += kWave) {
+    float x[3];
+    transform(M, lig[3 * j], lig[3 * j + 1], lig[3 * j + 2], x);
+    const float dx = x[0] - ref[3 * j], dy = x[1] - ref[3 * j + 1], dz = x[2] - ref[3 * j + 2];
+    acc += fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  }
+  const float s = wave_sum(acc);
+  if (lane == 0) out[p] = sqrtf(s / (float)nb);
+}
+
 __global__ void fill_int(int* p, int v) { *p = v; }
 
 // Per-(pose, bead) terms; one thread per bead-instance (parity / debugging path).
@@ -1444,6 +1465,18 @@ int atj_point_terms(const AtjGrid* grid, const AtjForceField* ff, const AtjRecep
   a.N = M;
   hipLaunchKernelGGL(bead_terms_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
                      nullptr, 1, coords, types, grid4, voxel, nbene, nbgrad);
+  HIPCHK(hipGetLastError());
+  return ATJ_OK;
+}
+
+int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float* ref, int32_t nb, float* out,
+                    void* stream) {
+  if (N < 0 || nb <= 0) return fail(ATJ_ERR_INVALID, "N < 0 or no beads");
+  if (N == 0) return ATJ_OK;
+  if (!poses || !lig || !ref || !out) return fail(ATJ_ERR_INVALID, "NULL argument");
+  constexpr int kWavesPerBlock = 4;
+  hipLaunchKernelGGL(ligand_rmsd_kernel, dim3((N + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kWave * kWavesPerBlock),
+                     0, (hipStream_t)stream, poses, N, lig, ref, nb, out);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }

@@ -197,6 +197,12 @@ int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor
 int atj_optimizer_step(const float* dEdM, float* energy, int32_t N, const AtjPoseState* state,
                        const AtjOptimizer* opt, void* stream);
 
+/* Ligand RMSD of N poses (reference: test-rmsd.py:13-17): poses [N][16] row-vector
+ * matrices applied to lig [nb][3], deviations from ref [nb][3]; out [N] =
+ * sqrt(sum_j |x'_j - ref_j|^2 / nb).  All device pointers. */
+int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float* ref, int32_t nb, float* out,
+                    void* stream);
+
 /* Adjoint of the pair term w.r.t. the pair table: for weights w[N]
  * (upstream dL/dE_pose) accumulates sum_pose w * dE_pose/d(rc, ac, emin) into
  * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated) and

@@ -1,0 +1,106 @@
+"""Writes the pose / structure I/O fixtures from data files the reference holds (run in a
+container that has the reference checkout; the fixtures are committed, this script is not
+needed at test time).
+
+  dat_excerpt.dat      first 12 structures of mono-leu/minim-sorted.dat (ATTRACT .dat text)
+  dat_pairs.npz        64 (receptor dofs, ligand dofs) -> 4x4 matrix pairs: .dat entries of
+                       mono-leu/minim-sorted.dat and the matrices mono-leu/minim-sorted.npy holds
+                       for them (about 8,000 of the 10,000 poses match exactly; the rest of the
+                       .npy comes from another run)
+  score_pairs.npz      the 10 poses test-grid-energrad.py prints (the first 10 matrices of
+                       minim-sorted.npy): JAX -dE/dM translation rows and torque blocks
+                       (test-grid-energrad.out) with, for the same poses found in
+                       minim-sorted.dat, their Euler angles and ATTRACT's Energy / Gradients
+                       lines (mono-leu/minim-sorted-grid.score); poses without a .dat entry
+                       are left out
+  (test-rmsd.lrmsd is not used: test-rmsd.py:7 asserts equal atom counts, but the arrays
+  it loads hold 1275 and 1244 atoms, so its inputs cannot be reconstructed from the files
+  the reference holds; RMSD is pinned by closed forms instead)
+  pdb_excerpt.pdb      first 40 ATOM records of mono-leu/mono-leuc.pdb, with
+  pdb_expected.npz     their coordinates and names from mono-leu/mono-leuc-pdb.npy
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def dat_structures(text):
+    """Per structure: list of dof rows (only what the fixture needs; the product parser
+    is attract_jax_amd.structio.read_dat)."""
+    out, cur = [], None
+    for line in text.split("\n"):
+        s = line.strip()
+        if s.startswith("#"):
+            if re.fullmatch(r"#\d+", s):
+                cur = []
+                out.append(cur)
+            continue
+        if s and cur is not None:
+            cur.append([float(x) for x in s.split()])
+    return np.array(out)
+
+
+def main():
+    dat_text = open(os.path.join(REF, "mono-leu/minim-sorted.dat")).read()
+    # excerpt: header + first 12 structures
+    lines = dat_text.split("\n")
+    starts = [i for i, l in enumerate(lines) if re.fullmatch(r"#\d+", l.strip())]
+    with open(os.path.join(OUT, "dat_excerpt.dat"), "w") as f:
+        f.write("\n".join(lines[:starts[12]]) + "\n")
+    P = dat_structures(dat_text)
+    M = np.load(os.path.join(REF, "mono-leu/minim-sorted.npy"), allow_pickle=False)
+    # pairs matched by the relative translation (exact to 1e-9); the tests check the rotation blocks
+    rel = P[:, 1, 3:] - P[:, 0, 3:]
+    key = {tuple(np.round(v, 4)): j for j, v in enumerate(rel)}
+    pairs = []
+    for i in range(len(M)):
+        j = key.get(tuple(np.round(M[i, 3, :3], 4)))
+        if j is not None and np.abs(rel[j] - M[i, 3, :3]).max() < 1e-9:
+            pairs.append((j, i))
+        if len(pairs) == 400:
+            break
+    rng = np.random.default_rng(0)
+    pick = rng.choice(len(pairs), 64, replace=False)
+    jj = np.array([pairs[k][0] for k in pick])
+    ii = np.array([pairs[k][1] for k in pick])
+    np.savez(os.path.join(OUT, "dat_pairs.npz"), rec=P[jj, 0], lig=P[jj, 1], mat=M[ii])
+
+    out_txt = open(os.path.join(REF, "test-grid-energrad.out")).read()
+    num = r"-?\d+\.\d+(?:e[+-]\d+)?"
+    tr = np.array([float(x) for x in re.findall(num, out_txt.split("Gradients (translational):")[1]
+                                                .split("Gradients (torque)")[0])])[:30].reshape(10, 3)
+    tq = np.array([float(x) for x in re.findall(num, out_txt.split("Gradients (torque):")[1])])[:90].reshape(10, 3, 3)
+    en = np.array([float(x) for x in re.findall(num, out_txt.split("Energies:")[1]
+                                                .split("Gradients")[0])])[:10]
+    sc = open(os.path.join(REF, "mono-leu/minim-sorted-grid.score")).read().split("\n")
+    E_att = np.array([float(l.split()[1]) for l in sc if l.strip().startswith("Energy:")])
+    G_att = np.array([[float(x) for x in l.split()[1:]] for l in sc if l.strip().startswith("Gradients:")])
+    # test-grid-energrad.py ran on the .npy poses: find each one's .dat / .score entry (poses of
+    # the other run have none and are left out)
+    jax_i, dat_of = [], []
+    for i in range(10):
+        j = key.get(tuple(np.round(M[i, 3, :3], 4)))
+        if j is not None and np.abs(rel[j] - M[i, 3, :3]).max() < 1e-9:
+            jax_i.append(i)
+            dat_of.append(j)
+    jax_i, dat_of = np.array(jax_i), np.array(dat_of)
+    np.savez(os.path.join(OUT, "score_pairs.npz"), euler=P[dat_of, 1, :3], neg_dEdt=tr[jax_i],
+             neg_dEdR=tq[jax_i], energy_jax=en[jax_i], energy_attract=E_att[dat_of],
+             gradients_attract=G_att[dat_of])
+
+    lig = np.load(os.path.join(REF, "mono-leu/mono-leuc-pdb.npy"), allow_pickle=False)
+    xyz = lambda a: np.stack([a["x"], a["y"], a["z"]], axis=1).astype(np.float64)
+    pdb_lines = [l for l in open(os.path.join(REF, "mono-leu/mono-leuc.pdb")).read().split("\n")
+                 if l.startswith("ATOM")][:40]
+    with open(os.path.join(OUT, "pdb_excerpt.pdb"), "w") as f:
+        f.write("\n".join(pdb_lines) + "\n")
+    np.savez(os.path.join(OUT, "pdb_expected.npz"), xyz=xyz(lig[:40]), name=lig["name"][:40].astype("U4"))
+
+
+if __name__ == "__main__":
+    main()
