@@ -27,6 +27,70 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _counter_per_dispatch(csv_path, counter):
+    """{dispatch id: (kernel name, summed counter value)} of a rocprofv3 counter CSV."""
+    import csv
+    out = {}
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            k = int(r["Dispatch_Id"])
+            name, v = out.get(k, (r["Kernel_Name"], 0.0))
+            out[k] = (name, v + float(r["Counter_Value"]))
+    return out
+
+
+def measure_traffic(args, wpb, opt):
+    """HBM traffic of one minimizer launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
+    timed pose-kernel dispatch of a one-launch child run of this workload, one rocprofv3 pass
+    per counter, each scaled by the value the same counter reads for a 1 GiB device copy in
+    the same run (1 GiB read, 1 GiB written).  Returns None when rocprofv3 is unavailable or a
+    pass fails."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
+             "--no-plain", "--serve-jobs", "0", "--no-traffic", "--copy-calibration", "--waves", str(wpb),
+             "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
+             str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    tmp = tempfile.mkdtemp(prefix="atj_traffic_", dir="/tmp")
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(tmp, ctr)
+        log(f"[bench] traffic pass {ctr}")
+        proc = subprocess.Popen([exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "c", "--", *child],
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env, cwd="/tmp",
+                                start_new_session=True)
+        try:
+            status = proc.wait(timeout=240)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, 9)  # the profiler and the profiled run
+            proc.wait()
+            log(f"[bench] traffic pass {ctr} timed out")
+            return None
+        # the profiler may exit non-zero after writing complete results; the CSV decides
+        files = glob.glob(os.path.join(d, "**", "c_counter_collection.csv"), recursive=True)
+        if not files:
+            log(f"[bench] traffic pass {ctr} failed (status {status})")
+            return None
+        disp = _counter_per_dispatch(files[0], ctr)
+        copy = [v for name, v in disp.values() if "copyBuffer" in name]
+        pose = [disp[k][1] for k in sorted(disp) if "pose_kernel" in disp[k][0]]
+        if not copy or not pose or max(copy) <= 0:
+            return None
+        scale = (2 ** 30 / 1024) / max(copy)  # KiB a 1 GiB copy should read (write), over what it reads
+        res[ctr] = (pose[-1] * scale * 1024, scale)
+    shutil.rmtree(tmp, ignore_errors=True)
+    return {"fetch_bytes": res["FETCH_SIZE"][0], "write_bytes": res["WRITE_SIZE"][0],
+            "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +110,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean job arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the minimizer's HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (poses shard trivially across "
@@ -146,6 +212,16 @@ def main():
                               "est_tflops": flops / (k_ms * 1e-3) / 1e12},
         "energy_finite_frac": float(np.isfinite(E_final).mean()),
     }
+
+    if not args.no_traffic:
+        tr = measure_traffic(args, system.launch.waves_per_block, system.launch.options)
+        if tr is not None:
+            result["roofline"]["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
+            result["roofline"]["traffic_detail"] = dict(
+                tr, method="rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of the timed pose-kernel dispatch of a "
+                           "one-launch run; KiB counts scaled by a 1 GiB device copy measured in the same run")
+            log(f"[bench] traffic: {result['roofline']['traffic'] / 1e9:.1f} GB per launch "
+                f"(algorithmic {algo_bytes / 1e9:.1f} GB)")
 
     if not args.no_plain:
         nl = min(args.plain_ligands, len(ligs))
