@@ -201,6 +201,7 @@ __device__ __forceinline__ void transform(const float M[16], float x, float y, f
 
 struct GridView {
   float ox, oy, oz, spacing, ge, p2;
+  float lx, ly, lz, lx2, ly2, lz2;  // dim - 1 as floats: the exclusive upper bounds of the inside tests
   int dx, dy, dz, dx2, dy2, dz2, ntypes, layout;
   unsigned slab, slab2;  // records (corner) or points (plain) per atom type, inner / outer
   const float* inner;
@@ -264,8 +265,7 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   const float vx = (xp[0] - G.ox) / G.spacing;
   const float vy = (xp[1] - G.oy) / G.spacing;
   const float vz = (xp[2] - G.oz) / G.spacing;
-  const bool inside = (vx >= 0.f) && (vx < (float)(G.dx - 1)) && (vy >= 0.f) && (vy < (float)(G.dy - 1)) &&
-                      (vz >= 0.f) && (vz < (float)(G.dz - 1));
+  const bool inside = (vx >= 0.f) && (vx < G.lx) && (vy >= 0.f) && (vy < G.ly) && (vz >= 0.f) && (vz < G.lz);
   E = 0.f;
   g[0] = g[1] = g[2] = 0.f;
   raw_out[0] = raw_out[1] = raw_out[2] = raw_out[3] = 0.f;
@@ -284,8 +284,8 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
     in_any = 1;
   } else if (G.outer != nullptr) {
     const float ox = (vx + G.ge) / 2.f, oy = (vy + G.ge) / 2.f, oz = (vz + G.ge) / 2.f;
-    const bool in_outer = (ox >= 0.f) && (ox < (float)(G.dx2 - 1)) && (oy >= 0.f) &&
-                          (oy < (float)(G.dy2 - 1)) && (oz >= 0.f) && (oz < (float)(G.dz2 - 1));
+    const bool in_outer =
+        (ox >= 0.f) && (ox < G.lx2) && (oy >= 0.f) && (oy < G.ly2) && (oz >= 0.f) && (oz < G.lz2);
     if (!in_outer) return;
     trilinear<LAYOUT>(G.outer, G.slab2, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
     if (exact) inv = 0.5f / G.spacing;
@@ -381,7 +381,7 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
   const float dv2 = fmaf(3.f, t1, rep) * (-2.f * (u2 * u2));
   const float r2 = fabsf(pa.z);
-  const float iv = pa.z < 0.f ? -1.f : 1.f;
+  const float iv = copysignf(1.f, pa.z);  // ivor rides on the sign of the rmin2 field
   const bool lo = dsq < r2;
   const float s = lo ? 1.f : iv;
   E = fmaf(s, vlj, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
@@ -1216,6 +1216,12 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.dz2 = g->dim2[2];
   G.ntypes = g->ntypes;
   G.layout = g->layout;
+  G.lx = (float)(g->dim[0] - 1);
+  G.ly = (float)(g->dim[1] - 1);
+  G.lz = (float)(g->dim[2] - 1);
+  G.lx2 = (float)(g->dim2[0] - 1);
+  G.ly2 = (float)(g->dim2[1] - 1);
+  G.lz2 = (float)(g->dim2[2] - 1);
   // per-type slab sizes; voxel and cell indices within a slab are 32-bit
   const bool corner = g->layout == ATJ_LAYOUT_CORNER;
   const unsigned long long pts = (unsigned long long)g->dim[0] * g->dim[1] * g->dim[2];
