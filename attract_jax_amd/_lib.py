@@ -36,10 +36,13 @@ class AtjGrid(ctypes.Structure):
         ("plateaudis2", ctypes.c_float),
         ("ntypes", ctypes.c_int32),
         ("layout", ctypes.c_int32),
+        ("nshell", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("inner", ctypes.c_void_p),
         ("outer", ctypes.c_void_p),
         ("nbgrid", ctypes.c_void_p),
         ("nblist", ctypes.c_void_p),
+        ("nbshell", ctypes.c_void_p),
     ]
 
 
@@ -138,6 +141,7 @@ SIGNATURES = {
                                   ctypes.c_int),
     "atj_grid_fill_neighbours": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjReceptor), _F, _P, _P, _P, _P, _P],
                                  ctypes.c_int),
+    "atj_grid_shell_lists": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjReceptor), _I, _P, _P, _P], ctypes.c_int),
     "atj_grid_pack_corners": ([_P, _I, ctypes.POINTER(ctypes.c_int32 * 3), _P, _P], ctypes.c_int),
 }
 

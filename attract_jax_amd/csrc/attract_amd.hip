@@ -20,6 +20,7 @@
 // transform and the voxelisation bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -44,6 +45,8 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kWave = 64;
 constexpr int kListPad = 16;       // lists are padded with 0xFFFF to a multiple of this
+constexpr int kMaxShell = 6;       // most list shells per grid point (16-bit counts in one int4)
+constexpr double kShellMargin = 1e-3;  // A added to every shell radius: covers FP32 rounding of distances
 constexpr int kMaxWavesPerBlock = 16;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
 constexpr int kPairCost = 30;      // approximate instructions per neighbour slot
@@ -203,11 +206,14 @@ struct GridView {
   float ox, oy, oz, spacing, ge, p2;
   float lx, ly, lz, lx2, ly2, lz2;  // dim - 1 as floats: the exclusive upper bounds of the inside tests
   int dx, dy, dz, dx2, dy2, dz2, ntypes, layout;
+  int nshell;            // 0: lists from nbgrid; > 0: shell-ordered lists, starts and lengths from nbshell
+  float shinv;           // 1 / shell width in cell units: 2 nshell / sqrt(3)
   unsigned slab, slab2;  // records (corner) or points (plain) per atom type, inner / outer
   const float* inner;
   const float* outer;
   const int2* nbgrid;
   const uint16_t* nblist;
+  const int4* nbshell;  // per grid point: the start, then 16-bit counts c0..c5 (uint16 words 2..7)
 };
 
 // Trilinear interpolation over one cell (test-grid-energrad.py:183-194):
@@ -256,12 +262,16 @@ __device__ __forceinline__ void trilinear(const float* g, unsigned slab, int t, 
   }
 }
 
-// Grid term of one bead at x'.  Returns E and dE/dx'; inner voxel index for the
-// neighbour lookup (-1 outside the inner grid, test-grid-energrad.py:216-218,246-253).
+// Grid term of one bead at x'.  Returns E and dE/dx', the inner voxel of the neighbour
+// lookup (-1 outside the inner grid, test-grid-energrad.py:216-218,246-253) and, with
+// do_lists, the start and length of the list the bead reads (0 outside): the voxel's whole
+// list, or with shell-ordered lists the leading entries of shell
+// min(floor(distance from the voxel's grid point / shell width), nshell - 1).  The list
+// entry is loaded before the grid corners so that both loads are in flight together.
 template <int LAYOUT>
 __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const float xp[3], bool do_grid,
-                                          bool exact, float& E, float g[3], int& voxel,
-                                          unsigned& in_any, float raw_out[4]) {
+                                          bool do_lists, bool exact, float& E, float g[3], int& voxel, int& cnt,
+                                          int& start, unsigned& in_any, float raw_out[4]) {
   const float vx = (xp[0] - G.ox) / G.spacing;
   const float vy = (xp[1] - G.oy) / G.spacing;
   const float vz = (xp[2] - G.oz) / G.spacing;
@@ -271,9 +281,28 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   raw_out[0] = raw_out[1] = raw_out[2] = raw_out[3] = 0.f;
   in_any = 0;
   voxel = -1;
+  cnt = start = 0;
   if (inside) {
-    const int ix = (int)floorf(vx + 0.5f), iy = (int)floorf(vy + 0.5f), iz = (int)floorf(vz + 0.5f);
+    const float wx = vx + 0.5f, wy = vy + 0.5f, wz = vz + 0.5f;
+    const float fx = floorf(wx), fy = floorf(wy), fz = floorf(wz);
+    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     voxel = (ix * G.dy + iy) * G.dz + iz;
+    if (do_lists) {
+      if (G.nshell > 0) {
+        // offset from the grid point in cell units (w - floor(w) is exact), |o| <= 1/2 per axis
+        const float ox = (wx - fx) - 0.5f, oy = (wy - fy) - 0.5f, oz = (wz - fz) - 0.5f;
+        const float d = __builtin_amdgcn_sqrtf(fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
+        const int shell = min((int)(d * G.shinv), G.nshell - 1);
+        // two loads from the entry's 16 bytes: the start (word 0) and count `shell` (uint16 2 + shell)
+        const int4* e = G.nbshell + voxel;
+        start = *reinterpret_cast<const int*>(e);
+        cnt = reinterpret_cast<const uint16_t*>(e)[2 + shell];
+      } else {
+        const int2 e = G.nbgrid[voxel];
+        cnt = e.x;
+        start = e.y;
+      }
+    }
   }
   if (!do_grid || G.inner == nullptr) return;
   float raw[4], dEdv[3];
@@ -625,7 +654,7 @@ __device__ float pose_energrad(const KArgs& a, const float4* __restrict__ s_rec,
       float E, g[3], raw[4];
       int voxel;
       unsigned in_any;
-      bead_grid<LAYOUT>(G, gt, xp, do_grid, EXACT, E, g, voxel, in_any, raw);
+      bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, E, g, voxel, cnt, start, in_any, raw);
       acc[0] += E;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -636,12 +665,7 @@ __device__ float pose_energrad(const KArgs& a, const float4* __restrict__ s_rec,
       st[0] += in_any;
       if (voxel >= 0) {
         st[1] += 1;
-        if (do_nb) {
-          const int2 e = G.nbgrid[voxel];
-          cnt = e.x;
-          start = e.y;
-          st[2] += (unsigned)cnt;
-        }
+        st[2] += (unsigned)cnt;
       }
     }
     if (!do_nb) continue;
@@ -990,22 +1014,22 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
   }
   const int gt = tb & 0xffff, lt = tb >> 16;
   float E, g[3], raw[4];
-  int voxel;
+  const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0, do_nb = (a.flags & ATJ_TERM_NB) != 0;
+  int voxel, cnt, start;
   unsigned in_any;
   if (a.G.layout == ATJ_LAYOUT_CORNER)
-    bead_grid<ATJ_LAYOUT_CORNER>(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
+    bead_grid<ATJ_LAYOUT_CORNER>(a.G, gt, xp, do_grid, do_nb, exact, E, g, voxel, cnt, start, in_any, raw);
   else
-    bead_grid<ATJ_LAYOUT_PLAIN>(a.G, gt, xp, (a.flags & ATJ_TERM_GRID) != 0, exact, E, g, voxel, in_any, raw);
+    bead_grid<ATJ_LAYOUT_PLAIN>(a.G, gt, xp, do_grid, do_nb, exact, E, g, voxel, cnt, start, in_any, raw);
   if (grid4) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) grid4[idx * 4 + i] = raw[i];
   }
   if (voxel_out) voxel_out[idx] = voxel;
   float e = 0.f, gg[3] = {0.f, 0.f, 0.f};
-  if (voxel >= 0 && (a.flags & ATJ_TERM_NB)) {
-    const int2 ent = a.G.nbgrid[voxel];
-    for (int c = 0; c < ent.x; ++c) {
-      const unsigned r = a.G.nblist[ent.y + c];
+  if (voxel >= 0 && do_nb) {
+    for (int c = 0; c < cnt; ++c) {
+      const unsigned r = a.G.nblist[start + c];
       const float4 rb = a.rec[r];
       const float ddx = xp[0] - rb.x, ddy = xp[1] - rb.y, ddz = xp[2] - rb.z;
       const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
@@ -1170,6 +1194,40 @@ __global__ void fill_neighbours_kernel(GridView G, const float4* rec, int nrec, 
   nbgrid[v] = make_int2(n, (int)s);
 }
 
+// Shell order of one grid point's list (atj_grid_shell_lists).  Entry r at distance d
+// from the point goes to group 0 when d < r0 (= plateaudis + margin), else to group
+// min(n + 1, ceil((d - r0) / h)) (h = shell width); groups in order, receptor order kept
+// within a group.  Count k of nbshell[v] = the entries in groups 0..k + 1: every entry
+// closer than r0 + (k + 1) h (the quotient rounds monotonically, so no such entry lands in
+// a later group).
+__device__ __forceinline__ int shell_group(float d, float r0, float h, int n) {
+  return d < r0 ? 0 : min(n + 1, (int)ceilf((d - r0) / h));
+}
+
+__global__ void shell_lists_kernel(GridView G, const float4* rec, int n, float r0, float h, int npts,
+                                   uint16_t* out, int4* nbshell) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= npts) return;
+  const int2 pl = G.nbgrid[v];
+  const int iz = v % G.dz, iy = (v / G.dz) % G.dy, ix = v / (G.dy * G.dz);
+  const float px = fmaf((float)ix, G.spacing, G.ox);
+  const float py = fmaf((float)iy, G.spacing, G.oy);
+  const float pz = fmaf((float)iz, G.spacing, G.oz);
+  int w = 0;
+  unsigned c[3] = {0u, 0u, 0u};
+  for (int j = 0; j <= n + 1; ++j) {
+    for (int t = 0; t < pl.x; ++t) {
+      const uint16_t r = G.nblist[pl.y + t];
+      const float4 rb = rec[r];
+      const float dx = rb.x - px, dy = rb.y - py, dz = rb.z - pz;
+      if (shell_group(sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), r0, h, n) == j) out[pl.y + w++] = r;
+    }
+    const int k = j - 1;  // shell k reads groups 0..k + 1
+    if (k >= 0 && k < n) c[k >> 1] |= (unsigned)w << (16 * (k & 1));
+  }
+  nbshell[v] = make_int4(pl.y, (int)c[0], (int)c[1], (int)c[2]);
+}
+
 __global__ void pack_corners_kernel(const float* plain, int ntypes, int dx, int dy, int dz, float* packed) {
   const long long nrec = (long long)ntypes * (dx - 1) * (dy - 1) * (dz - 1);
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1202,6 +1260,7 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
     for (int i = 0; i < 3; ++i)
       if (g->dim2[i] < 2) return fail(ATJ_ERR_INVALID, "outer grid dim2 must be >= 2");
   if (need_lists && (!g->nbgrid || !g->nblist)) return fail(ATJ_ERR_INVALID, "neighbour lists are NULL");
+  if (need_lists && g->nshell > 0 && !g->nbshell) return fail(ATJ_ERR_INVALID, "nbshell is NULL");
   G.ox = g->origin[0];
   G.oy = g->origin[1];
   G.oz = g->origin[2];
@@ -1216,6 +1275,9 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.dz2 = g->dim2[2];
   G.ntypes = g->ntypes;
   G.layout = g->layout;
+  if (g->nshell < 0 || g->nshell > kMaxShell) return fail(ATJ_ERR_INVALID, "nshell must be 0..6");
+  G.nshell = g->nshell;
+  G.shinv = (float)(2.0 * g->nshell / std::sqrt(3.0));
   G.lx = (float)(g->dim[0] - 1);
   G.ly = (float)(g->dim[1] - 1);
   G.lz = (float)(g->dim[2] - 1);
@@ -1239,6 +1301,7 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.outer = g->outer;
   G.nbgrid = reinterpret_cast<const int2*>(g->nbgrid);
   G.nblist = g->nblist;
+  G.nbshell = reinterpret_cast<const int4*>(g->nbshell);
   return ATJ_OK;
 }
 
@@ -1521,6 +1584,25 @@ int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff, cons
                        ff->nlt, lig_fftype_dev, c2, outer_out, G.dx2, G.dy2, G.dz2, 1);
     HIPCHK(hipGetLastError());
   }
+  return ATJ_OK;
+}
+
+int atj_grid_shell_lists(const AtjGrid* grid, const AtjReceptor* rec, int32_t nshell, uint16_t* nblist_out,
+                         int32_t* nbshell_out, void* stream) {
+  if (nshell < 1 || nshell > kMaxShell) return fail(ATJ_ERR_INVALID, "nshell must be 1..6");
+  if (!grid || grid->nshell != 0) return fail(ATJ_ERR_INVALID, "shell order is built from lists read whole (nshell 0)");
+  GridView G;
+  int rc = make_view(grid, G, true);
+  if (rc) return rc;
+  if (!rec || (rec->nrec > 0 && !rec->beads) || !nblist_out || !nbshell_out)
+    return fail(ATJ_ERR_INVALID, "NULL argument");
+  const long long npts = (long long)G.dx * G.dy * G.dz;
+  const double h = (double)grid->spacing * std::sqrt(3.0) / (2.0 * nshell);
+  const float r0 = (float)((double)grid->plateaudis + kShellMargin);
+  hipLaunchKernelGGL(shell_lists_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream, G,
+                     reinterpret_cast<const float4*>(rec->beads), (int)nshell, r0, (float)h, (int)npts, nblist_out,
+                     reinterpret_cast<int4*>(nbshell_out));
+  HIPCHK(hipGetLastError());
   return ATJ_OK;
 }
 

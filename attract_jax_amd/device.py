@@ -186,6 +186,8 @@ class DeviceGrid:
         self.outer_plain = outer_plain
         self.nbgrid = nbgrid
         self.nblist = nblist
+        self.nshell = 0
+        self.nbshell = None
         self.inner = self._relayout(inner_plain, self.dim)
         self.outer = self._relayout(outer_plain, self.dim2) if outer_plain is not None else None
         if not keep_plain and layout != _lib.ATJ_LAYOUT_PLAIN:
@@ -211,9 +213,12 @@ class DeviceGrid:
 
     def nbytes(self):
         return sum(t.numel() * t.element_size() for t in (self.inner, self.outer, self.nbgrid, self.nblist,
+                                                          self.nbshell,
                                                           self.inner_plain, self.outer_plain) if t is not None)
 
-    def struct(self):
+    def struct(self, whole_lists=False):
+        """AtjGrid of this grid; with whole_lists the kernels read every list entry even when
+        shell_lists made shell counts."""
         g = _lib.AtjGrid()
         g.origin[:] = [float(v) for v in self.origin]
         g.spacing = self.spacing
@@ -228,7 +233,45 @@ class DeviceGrid:
         g.outer = self.outer.data_ptr() if self.outer is not None else 0
         g.nbgrid = self.nbgrid.data_ptr()
         g.nblist = self.nblist.data_ptr()
+        if self.nshell > 0 and not whole_lists:
+            g.nshell = self.nshell
+            g.nbshell = self.nbshell.data_ptr()
         return g
+
+    def shell_lists(self, receptor, nshell):
+        """Reorder every list into distance shells around its grid point (nshell 1..6; 0
+        keeps the current order and makes beads read whole lists again).
+
+        A bead in the cell of point v is within spacing*sqrt(3)/2 of v; with shell width
+        h = spacing*sqrt(3)/(2*nshell) and k = floor(|x - v| / h), only list entries with
+        |r - v| < plateaudis + (k + 1) h can lie within plateaudis of it.  The lists are
+        reordered by that bound (receptor order kept among equals) and nbshell[v] holds the
+        list start and the number of leading entries a bead of each shell reads, so pairs beyond the plateau
+        are mostly not visited; the pairs that contribute are the same, the summation order
+        differs.  Counts, starts and list memory are unchanged (host_lists and
+        to_reference_grid return the reordered lists, still valid ATTRACT lists)."""
+        nshell = int(nshell)
+        if nshell == 0:
+            self.nshell, self.nbshell, self._shell_rec = 0, None, None
+            return self
+        lib = _lib.load()
+        V = self.dim[0] * self.dim[1] * self.dim[2]
+        out = self.nblist.clone()
+        nbshell = torch.empty((V, 4), dtype=torch.int32, device=self.device)
+        _lib.check(lib.atj_grid_shell_lists(ctypes.byref(self.struct(whole_lists=True)),
+                                            ctypes.byref(receptor.struct()), nshell, _ptr(out), _ptr(nbshell),
+                                            _stream(self.device)))
+        self.nblist, self.nbshell, self.nshell = out, nbshell, nshell
+        self._shell_rec = receptor.beads
+        return self
+
+    def shells_valid_for(self, receptor):
+        """True when the shell counts were computed for these receptor coordinates."""
+        r = getattr(self, "_shell_rec", None)
+        if r is None or self.nshell == 0:
+            return False
+        b = receptor.beads
+        return r is b or (r.shape == b.shape and bool(torch.equal(r, b)))
 
     @classmethod
     def from_grid(cls, grid, layout=_lib.ATJ_LAYOUT_CORNER, device=None, keep_plain=False):
@@ -277,6 +320,8 @@ class DeviceGrid:
         self.neighbourdis = float(neighbourdis)
         self.ntypes = ntypes
         self.layout = layout
+        self.nshell = 0
+        self.nbshell = None
         self.inner = self.outer = None
         self.inner_plain = self.outer_plain = None
         V = self.dim[0] * self.dim[1] * self.dim[2]

@@ -19,6 +19,7 @@ from . import _lib
 from .device import DeviceGrid, ForceField, Ligand, Receptor, _ptr, _stream
 
 _grid_cache = {}
+DEFAULT_NSHELL = 6  # distance shells of the neighbour lists a ScoringSystem reads (0: whole lists)
 
 
 def device_grid(grid, layout=_lib.ATJ_LAYOUT_CORNER):
@@ -80,7 +81,8 @@ def point_terms(coords, grid_types, ff_types, grid, ff=None, receptor=None, flag
     voxel = torch.empty(M, dtype=torch.int32, device=d)
     nbe = torch.zeros(M, dtype=torch.float32, device=d)
     nbg = torch.zeros((M, 3), dtype=torch.float32, device=d)
-    _lib.check(lib.atj_point_terms(ctypes.byref(dg.struct()), ctypes.byref(ffd.struct()),
+    whole = dg.nshell > 0 and not dg.shells_valid_for(receptor)  # shells of other coordinates: whole lists
+    _lib.check(lib.atj_point_terms(ctypes.byref(dg.struct(whole_lists=whole)), ctypes.byref(ffd.struct()),
                                    ctypes.byref(receptor.struct()), _ptr(c), _ptr(types), M, _ptr(grid4),
                                    _ptr(voxel), _ptr(nbe), _ptr(nbg), flags, _stream(d)))
     return grid4, voxel, nbe, nbg
@@ -119,8 +121,13 @@ class ScoringSystem:
     """Receptor grid, pair table and receptor beads resident on the device, plus a
     work-counter workspace: the state the energrad / minimize launches share."""
 
-    def __init__(self, grid, ff, receptor, flags=None, launch=None):
+    def __init__(self, grid, ff, receptor, flags=None, launch=None, nshell=DEFAULT_NSHELL):
+        """nshell > 0 puts the grid's lists in distance shells for this receptor (same
+        contributing pairs, fewer list entries read per bead) unless they already are; a
+        grid in shell order for other receptor coordinates is read whole."""
         self.grid = device_grid(grid)
+        if nshell and self.grid.nshell == 0:
+            self.grid.shell_lists(receptor, nshell)
         self.ff = device_ff(ff)
         self.receptor = receptor
         self.device = self.grid.device
@@ -128,14 +135,18 @@ class ScoringSystem:
         self.launch = launch or _lib.AtjLaunch(8, 0, 0)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.stats = torch.zeros(_lib.ATJ_NSTATS, dtype=torch.int64, device=self.device)
-        self._gs = self.grid.struct()
-        self._fs = self.ff.struct()
-        self._rs = receptor.struct()
+        self.refresh()
 
     def refresh(self):
-        self._gs = self.grid.struct()
+        """Re-read the device pointers of grid, pair table and receptor (after any of them
+        was replaced); the tensors they point to are kept alive with the structs."""
+        g = self.grid
+        whole = g.nshell > 0 and not g.shells_valid_for(self.receptor)
+        self._gs = g.struct(whole_lists=whole)
         self._fs = self.ff.struct()
         self._rs = self.receptor.struct()
+        self._alive = (g.inner, g.outer, g.nbgrid, g.nblist, None if whole else g.nbshell, self.receptor.beads,
+                       self.ff)
 
     def jobs_tensor(self, ligands, nsteps=0, lr_rot=1e-3, lr_trans=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
                     pose_counts=None, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):

@@ -197,8 +197,9 @@ def serving_jobs(njobs, seed=5000, rate=20.0):
     return jobs
 
 
-def build_system(w, device=None, layout=None, keep_plain=False, flags=None, launch=None):
-    """Device objects of a workload: (ScoringSystem, [Ligand]); the grid is computed on the GPU."""
+def build_system(w, device=None, layout=None, keep_plain=False, flags=None, launch=None, nshell=None):
+    """Device objects of a workload: (ScoringSystem, [Ligand]); the grid is computed on the GPU
+    (neighbour lists in nshell distance shells, ScoringSystem's default when None; 0 = whole lists)."""
     from . import _lib
     from .device import DeviceGrid, ForceField, Ligand, Receptor
     from .score import ScoringSystem
@@ -209,7 +210,8 @@ def build_system(w, device=None, layout=None, keep_plain=False, flags=None, laun
     grid = DeviceGrid.build(rec, ff, np.arange(NTYPES), layout=layout, device=device, keep_plain=keep_plain,
                             **w.grid_kw)
     ligs = [Ligand.create(l.coor, l.types - 1, l.types - 1, device) for l in w.ligands]
-    return ScoringSystem(grid, ff, rec, flags=flags, launch=launch), ligs
+    kw = {} if nshell is None else dict(nshell=nshell)
+    return ScoringSystem(grid, ff, rec, flags=flags, launch=launch, **kw), ligs
 
 
 def c5_receptors(n=16, seed=5000):

@@ -57,7 +57,7 @@ def measure_traffic(args, wpb, opt):
     child = [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
              "--no-plain", "--serve-jobs", "0", "--no-traffic", "--copy-calibration", "--waves", str(wpb),
              "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
-             str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps)]
+             str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps), "--nshell", str(args.nshell)]
     env = dict(os.environ, TMPDIR="/tmp")
     tmp = tempfile.mkdtemp(prefix="atj_traffic_", dir="/tmp")
     res = {}
@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0, help="waves per block (0 = autotuned)")
     ap.add_argument("--launch-options", type=int, default=0, help="ATJ_LAUNCH_* bits when --waves is given")
+    ap.add_argument("--nshell", type=int, default=6, help="distance shells of the neighbour lists (0 = whole lists)")
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the serving run (0 = skip)")
     ap.add_argument("--serve-receptors", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean job arrivals per second")
@@ -136,7 +137,7 @@ def main():
     w = c3(args.ligands, args.poses_per_ligand, args.adam_steps)
     log(f"[bench] workload generated in {time.time() - t0:.1f}s: {w.name}")
     t0 = time.time()
-    system, ligs = build_system(w, dev, keep_plain=True)
+    system, ligs = build_system(w, dev, keep_plain=True, nshell=args.nshell)
     torch.cuda.synchronize()
     log(f"[bench] grid built on the GPU in {time.time() - t0:.2f}s ({system.grid.nbytes() / 1e9:.2f} GB resident)")
     plan = BatchPlan(system, ligs, w.pose_counts, w.nsteps, w.lr_rot, w.lr_trans)
@@ -200,7 +201,7 @@ def main():
         "config": {"workload": w.name, "poses": plan.N, "ligands": args.ligands, "adam_steps": w.nsteps,
                    "receptor_beads": len(w.coor_rec), "grid": "inner 96^3 @0.9A + outer 64^3 @1.8A, 32 types",
                    "layout": "corner-packed" if system.grid.layout == _lib.ATJ_LAYOUT_CORNER else "plain",
-                   "waves_per_block": system.launch.waves_per_block,
+                   "waves_per_block": system.launch.waves_per_block, "list_shells": system.grid.nshell,
                    "receptor_path": "L1" if system.launch.options & _lib.ATJ_LAUNCH_REC_GLOBAL else "LDS"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

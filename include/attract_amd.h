@@ -58,6 +58,10 @@ typedef struct AtjGrid {
   float plateaudis2;      /* plateaudis**2 rounded once to float32 (test-grid-energrad.py:113) */
   int32_t ntypes;        /* number of potential grids (ligand grid types) */
   int32_t layout;         /* ATJ_LAYOUT_* of inner and outer */
+  int32_t nshell;         /* 0: a bead reads its point's whole list (nbgrid).  1..6: the lists are in
+                             shell order (atj_grid_shell_lists) and a bead of distance shell k
+                             from point v reads nbshell[v]'s count k from nbshell[v]'s start */
+  int32_t reserved;
   const float* inner;     /* potentials + gradients, NULL for a neighbour-only grid */
   const float* outer;     /* NULL = no outer grid */
   const int32_t* nbgrid;  /* [dx*dy*dz][2] = (count, start): 0-based start into nblist,
@@ -65,6 +69,8 @@ typedef struct AtjGrid {
                              alignment, start % 32 == 0, saves memory lines) */
   const uint16_t* nblist; /* receptor bead indices; each list padded with 0xFFFF to a multiple
                              of 16 entries; entries 0..15 are 0xFFFF */
+  const int32_t* nbshell; /* [dx*dy*dz][4] when nshell > 0, else unused: (start, c0 | c1 << 16,
+                             c2 | c3 << 16, c4 | c5 << 16), ck the list length shell k reads */
 } AtjGrid;
 
 /* ATTRACT pair table, compacted to receptor types x ligand types
@@ -224,6 +230,23 @@ int atj_reduce_partials(const float* partial, int32_t blocks, int32_t n, float* 
 int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff,
                               const AtjReceptor* rec, const int32_t* lig_fftype_dev,
                               float cutoff, float* inner_out, float* outer_out, void* stream);
+/* Shell order of the neighbour lists (no counterpart in the reference: a reordering of its
+ * lists, test-grid-energrad.py:246-253, that lets a bead skip entries beyond the plateau).
+ * Bead x in point v's cell (the beads whose list is v's) lies within spacing * sqrt(3) / 2
+ * of v; split that range into nshell shells of width h = spacing * sqrt(3) / (2 nshell), and
+ * let k = min(floor(|x - v| / h), nshell - 1) (nshell 1..6).  A receptor bead r of v's list can be within
+ * plateaudis of x only if |r - v| < plateaudis + (k + 1) h.  This call writes nblist_out, a
+ * copy of grid->nblist (same size; the caller copies it), with each list reordered into
+ * groups of increasing |r - v| (group 0: |r - v| < r0 = plateaudis + 0.001; group j:
+ * ceil((|r - v| - r0) / h) = j, clamped to nshell + 1; receptor order within a group), and
+ * nbshell_out [dx*dy*dz][4] = (start, c0 | c1 << 16, c2 | c3 << 16, c4 | c5 << 16), ck =
+ * the entries of groups 0..k+1 of the list.  The pairs
+ * within plateaudis of every bead are those of the full lists (the 0.001 A covers float
+ * rounding); the evaluated pair set is unchanged, the summation order is not.  grid->nshell
+ * must be 0; the result is used with nblist = nblist_out, nbshell = nbshell_out, nshell. */
+int atj_grid_shell_lists(const AtjGrid* grid, const AtjReceptor* rec, int32_t nshell, uint16_t* nblist_out,
+                         int32_t* nbshell_out, void* stream);
+
 /* Neighbour lists: receptor beads with |p - r| < neighbourdis of every inner grid point p.
  * Pass 1 writes counts [dx*dy*dz]; the caller turns round-up-to-16 counts into 0-based
  * starts (and fills entries 0..15 with 0xFFFF); pass 2 writes nbgrid [V][2] and nblist

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_c_layout():
-    assert ctypes.sizeof(_lib.AtjGrid) == 4 * 3 + 4 + 12 + 12 + 4 * 5 + 4 + 8 * 4
+    assert ctypes.sizeof(_lib.AtjGrid) == 4 * 3 + 4 + 12 + 12 + 4 * 5 + 4 + 8 + 8 * 5
     assert ctypes.sizeof(_lib.AtjOptimizer) == 4 * 8
     assert ctypes.sizeof(_lib.AtjJob) == 16 + 4 * 4 + 4 * 8
     assert ctypes.sizeof(_lib.AtjLigand) == 16
