@@ -204,6 +204,7 @@ __device__ __forceinline__ void transform(const float M[16], float x, float y, f
 
 struct GridView {
   float ox, oy, oz, spacing, ge, p2;
+  float rspacing;                   // RN(1 / spacing), for div_rn
   float lx, ly, lz, lx2, ly2, lz2;  // dim - 1 as floats: the exclusive upper bounds of the inside tests
   int dx, dy, dz, dx2, dy2, dz2, ntypes, layout;
   int nshell;            // 0: lists from nbgrid; > 0: shell-ordered lists, starts and lengths from nbshell
@@ -215,6 +216,19 @@ struct GridView {
   const uint16_t* nblist;
   const int4* nbshell;  // per grid point: the start, then 16-bit counts c0..c5 (uint16 words 2..7)
 };
+
+// a / b rounded to nearest, as IEEE division gives it, from y = RN(1/b) in three operations
+// (Markstein's correction step: with y within half an ulp of 1/b and q = RN(a y) within
+// one ulp of a/b, r = a - b q is exact in one fma and RN(q + r y) = RN(a/b)).  It needs r
+// not to underflow: callers take a true division for |a| < 2^-99 (checked bit for bit
+// against IEEE division for every float a in [2^-99, 4096) and several divisors).  Used for
+// the voxel coordinates (x' - origin) / spacing of test-grid-energrad.py:216-218, whose
+// voxel and trilinear weights must match a true division bit for bit.
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+  const float q = a * y;
+  const float r = fmaf(-q, b, a);
+  return fmaf(r, y, q);
+}
 
 // Trilinear interpolation over one cell (test-grid-energrad.py:183-194):
 // x0 = floor(v), x1 = ceil(v), w0 = x1 - v, w1 = v - x0; corners x -> y -> z.
@@ -272,9 +286,15 @@ template <int LAYOUT>
 __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const float xp[3], bool do_grid,
                                           bool do_lists, bool exact, float& E, float g[3], int& voxel, int& cnt,
                                           int& start, unsigned& in_any, float raw_out[4]) {
-  const float vx = (xp[0] - G.ox) / G.spacing;
-  const float vy = (xp[1] - G.oy) / G.spacing;
-  const float vz = (xp[2] - G.oz) / G.spacing;
+  const float ax = xp[0] - G.ox, ay = xp[1] - G.oy, az = xp[2] - G.oz;
+  float vx = div_rn(ax, G.spacing, G.rspacing);
+  float vy = div_rn(ay, G.spacing, G.rspacing);
+  float vz = div_rn(az, G.spacing, G.rspacing);
+  if (fminf(fminf(fabsf(ax), fabsf(ay)), fabsf(az)) < 0x1p-99f) {  // zero or tiny: r may underflow
+    vx = ax / G.spacing;
+    vy = ay / G.spacing;
+    vz = az / G.spacing;
+  }
   const bool inside = (vx >= 0.f) && (vx < G.lx) && (vy >= 0.f) && (vy < G.ly) && (vz >= 0.f) && (vz < G.lz);
   E = 0.f;
   g[0] = g[1] = g[2] = 0.f;
@@ -356,15 +376,15 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, ivor * rmin2, c_lo) and 8-byte side record (c_hi, gP) of
+// 16-byte pair record (rc, ac, ivor * rmin2, c_lo) and 8-byte side record (c_hi, gh) of
 // one pair type.  ivor = +-1 rides on the sign of rmin2 (ATTRACT's ipon is +-1,
 // attract-original/convert-attract-par.py:14-15); an ivor of another magnitude poisons
 // the record (NaN energies) rather than being misread.  The pair energy is
-// s vlj + (dsq < rmin2 ? c_lo : c_hi) and its gradient factor s dv2 + gP / sqrt(dsq), so
-// everything that depends on the pair type only is folded in here: c_lo = (ivor - 1) emin
-// and c_hi = 0 without the plateau; with it both carry -E(plateaudis) and gP the
-// force-shift factor -2 P dE/ddsq (P) (test-grid-energrad.py:124-147), zero for the exact
-// derivative.
+// s vlj + (dsq < rmin2 ? c_lo : c_hi) and its gradient factor -2 (s dh + gh / sqrt(dsq))
+// (see pair_eval), so everything that depends on the pair type only is folded in here:
+// c_lo = (ivor - 1) emin and c_hi = 0 without the plateau; with it both carry
+// -E(plateaudis) and gh = P dE/ddsq (P), the force-shift factor -2 P dE/ddsq (P)
+// (test-grid-energrad.py:124-147) divided by -2; zero for the exact derivative.
 struct PairType {
   float4 rec;
   float2 side;
@@ -384,18 +404,19 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
     const float EP = fmaf(sP, vljP, loP ? w : 0.f);
     t.rec.w = w - EP;
     t.side.x = -EP;
-    if (!exact) t.side.y = -(sP * fmaf(rc, pc.k1, ac * pc.k2));
+    if (!exact) t.side.y = 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
   }
   return t;
 }
 
-// ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and 2 x its dsq-derivative,
-// with the pair-type terms of the record (plateau subtraction and force shift when the
-// record carries them).  Returns E and gs such that dE/dx'_lig = gs * (x'_lig - x_rec).
-// The all-zero record gives E = gs = 0 for any finite dsq > 0 (padding and pairs beyond
-// the plateau).
+// ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and its dsq-derivative, with
+// the pair-type terms of the record (plateau subtraction and force shift when the record
+// carries them).  Returns E and h such that dE/dx'_lig = -2 h (x'_lig - x_rec):
+// h = -s d vlj / d dsq (+ gh / sqrt(dsq)); the factor -2 is applied by the caller once per
+// list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
+// pairs beyond the plateau).
 template <bool PLATEAU, bool EXACT>
-__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& gs) {
+__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& h) {
   float u, rs = 0.f;
   if (PLATEAU && !EXACT) {
     rs = __builtin_amdgcn_rsqf(dsq);
@@ -408,26 +429,30 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   const float t1 = rep - pa.y;
   const float vlj = t1 * (u2 * u);
   // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
-  const float dv2 = fmaf(3.f, t1, rep) * (-2.f * (u2 * u2));
+  const float dh = fmaf(3.f, t1, rep) * (u2 * u2);
   const float r2 = fabsf(pa.z);
   const float iv = copysignf(1.f, pa.z);  // ivor rides on the sign of the rmin2 field
   const bool lo = dsq < r2;
   const float s = lo ? 1.f : iv;
   E = fmaf(s, vlj, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
-  gs = s * dv2;
-  if (PLATEAU && !EXACT) gs = fmaf(ps.y, rs, gs);
+  h = s * dh;
+  if (PLATEAU && !EXACT) h = fmaf(ps.y, rs, h);
 }
 
+// pair_eval with run-time flags, returning the gradient factor gs = -2 h:
+// dE/dx'_lig = gs (x'_lig - x_rec).
 __device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const PairType& t, float& E,
                                              float& gs) {
+  float h;
   if (plateau) {
     if (exact)
-      pair_eval<true, true>(dsq, t.rec, t.side, E, gs);
+      pair_eval<true, true>(dsq, t.rec, t.side, E, h);
     else
-      pair_eval<true, false>(dsq, t.rec, t.side, E, gs);
+      pair_eval<true, false>(dsq, t.rec, t.side, E, h);
   } else {
-    pair_eval<false, false>(dsq, t.rec, t.side, E, gs);
+    pair_eval<false, false>(dsq, t.rec, t.side, E, h);
   }
+  gs = -2.f * h;
 }
 
 // Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs.
@@ -582,12 +607,12 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
-        float E, gs;
-        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, gs);
+        float E, h;
+        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, h);
         e += E;
-        gx = fmaf(gs, dd[cc][0], gx);
-        gy = fmaf(gs, dd[cc][1], gy);
-        gz = fmaf(gs, dd[cc][2], gz);
+        gx = fmaf(h, dd[cc][0], gx);
+        gy = fmaf(h, dd[cc][1], gy);
+        gz = fmaf(h, dd[cc][2], gz);
         npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
         if (MODE == kAdjoint && ok[cc]) {
           float dp[3];
@@ -607,7 +632,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       }
     }
     acc[0] += e;
-    const float g[3] = {gx, gy, gz};
+    const float g[3] = {-2.f * gx, -2.f * gy, -2.f * gz};
     const float xb[3] = {bl.x, bl.y, bl.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -1265,6 +1290,7 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.oy = g->origin[1];
   G.oz = g->origin[2];
   G.spacing = g->spacing;
+  G.rspacing = 1.f / g->spacing;  // IEEE single division: correctly rounded
   G.ge = g->gridextension;
   G.p2 = g->plateaudis2;
   G.dx = g->dim[0];
