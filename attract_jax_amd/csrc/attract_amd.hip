@@ -45,6 +45,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kWave = 64;
 constexpr int kListPad = 16;       // lists are padded with 0xFFFF to a multiple of this
+constexpr int kHalf = 32;          // ligands of at most this many beads take half a wave in the bead phase
 constexpr int kMaxShell = 6;       // most list shells per grid point (16-bit counts in one int4)
 constexpr double kShellMargin = 1e-3;  // A added to every shell radius: covers FP32 rounding of distances
 constexpr int kMaxWavesPerBlock = 16;
@@ -480,6 +481,8 @@ struct WaveScratch {
   float4 xp[kWave];  // x', y', z', ligand ff type (int bits)
   float4 xl[kWave];  // x, y, z (local), list start - S * first item (int bits)
   int mark[kWave];   // bead whose first list segment is item base + i of the current window
+  float M[16];       // pose matrix of a single-pose evaluation (12 used: rotation rows, translation)
+  float gridB[kAcc][kHalf];  // grid sums of the second pose of a shared bead phase
 };
 
 // Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
@@ -487,6 +490,8 @@ struct WaveScratch {
 // slot).
 constexpr int kBatch = 8;
 struct BatchScratch {
+  const float4* lig[kBatch];  // ligand beads of the slot's job
+  int nb[kBatch];             // and their number
   float M[kBatch][12];    // rotation rows and translation for the next evaluation
   float acc[kBatch][16];  // reduced sums of the last evaluation (kAcc used)
   float prm[kBatch][7];   // quaternion (x, y, z, w), translation
@@ -643,38 +648,62 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   }
 }
 
-// Energy and dE/dM of one pose (uniform pose matrix M).  Returns the wave-reduced sums in
-// the transposed layout of wave_sum_all: lane l holds accumulator acc_index(l)
-// (0: energy, 1 + 3k + l: dE/dM[k][l], 10 + l: dE/dM[3][l]).
+// One pose of an evaluation: ligand beads, pose matrix in LDS (rows 0-2 of the rotation,
+// then the translation: 12 floats) and adjoint weight.
+struct PoseIn {
+  const float4* lig;
+  const float* M;
+  int nb;
+  float wgt;
+};
+
+// Energy and dE/dM of pose A, or of poses A and B.  Returns the wave-reduced sums in the
+// transposed layout of wave_sum_all: lane l holds accumulator acc_index(l) (0: energy,
+// 1 + 3k + l: dE/dM[k][l], 10 + l: dE/dM[3][l]) of A in uA and of B in uB.
+// A pose of more than kHalf beads (B.nb must be 0) takes the whole wave, in chunks of 64
+// beads.  A pose of at most kHalf beads puts bead j on lane j; with a second such pose B its
+// beads run on lanes 32 + j in the same bead phase, and its grid sums wait in LDS while A's
+// pairs are evaluated, then move to lanes 0-31.  Either way a pose's sums are formed on the
+// same lanes in the same order (grid terms of bead j on lane j, then its pairs in windows of
+// its own, one reduction), so its result is the same alone or beside another.
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
-__device__ float pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                               const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws,
-                               const float4* __restrict__ lig, int nb, const float Min[16], float wgt,
-                               unsigned st[4], int lane) {
+__device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
+                              const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws, const PoseIn& A,
+                              const PoseIn& B, unsigned st[4], int lane, float& uA, float& uB) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
-  float M[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) M[i] = uni(Min[i]);
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
   const GridView& G = a.G;
+  const bool halves = A.nb <= kHalf;
+  const int nposes = halves && B.nb > 0 ? 2 : 1;
+  const int h = halves ? lane >> 5 : 0;  // pose of this lane in the bead phase
   float acc[kAcc];
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
+  uA = uB = 0.f;
 
-  for (int c0 = 0; c0 < nb; c0 += kWave) {
-    const int j = c0 + lane;
+  for (int c0 = 0; c0 < A.nb; c0 += kWave) {
+    const int j = halves ? lane & (kHalf - 1) : c0 + lane;
+    const int nbh = h ? B.nb : A.nb;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
     int lt = 0;
-    if (j < nb) {
-      const float4 b = lig[j];
+    if (j < nbh) {
+      const float4 b = (h ? B.lig : A.lig)[j];
       xl[0] = b.x;
       xl[1] = b.y;
       xl[2] = b.z;
       const int tb = __float_as_int(b.w);
       const int gt = tb & 0xffff;
       lt = tb >> 16;
+      const float* mp = h ? B.M : A.M;
+      float M[16];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) M[4 * k + l] = mp[3 * k + l];
+        M[4 * k + 3] = 1.f;
+      }
       transform(M, xl[0], xl[1], xl[2], xp);
       float E, g[3], raw[4];
       int voxel;
@@ -693,36 +722,68 @@ __device__ float pose_energrad(const KArgs& a, const float4* __restrict__ s_rec,
         st[2] += (unsigned)cnt;
       }
     }
-    if (!do_nb) continue;
-    // segment size per chunk: 8 list entries (fewer windows) or 4 (fewer padding slots)
-    const int seg8 = (cnt + 7) >> 3, seg4 = (cnt + 3) >> 2;
-    const int incl8 = wave_incl_scan(seg8);
-    const int incl4 = wave_incl_scan(seg4);
-    const int tot8 = __builtin_amdgcn_readlane(incl8, kWave - 1);
-    const int tot4 = __builtin_amdgcn_readlane(incl4, kWave - 1);
-    if (tot8 == 0) continue;
-    const int cost8 = ((tot8 + kWave - 1) / kWave) * (8 * kPairCost + kItemCost);
-    const int cost4 = ((tot4 + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
-    const bool use8 = cost8 <= cost4;
-    const int segs = use8 ? seg8 : seg4;
-    const int excl = (use8 ? incl8 : incl4) - segs;
-    ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-    ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
+    if (nposes == 2) {  // B's grid sums wait in LDS; its lanes start A's pair sums from zero
+      if (h) {
+#pragma unroll
+        for (int i = 0; i < kAcc; ++i) ws->gridB[i][lane - kHalf] = acc[i];
+      }
+#pragma unroll
+      for (int i = 0; i < kAcc; ++i) acc[i] = h ? 0.f : acc[i];
+    }
+    // segment size per pose and chunk: 8 list entries (fewer windows) or 4 (fewer padding slots)
+    int t8A = 0, t4A = 0, t8B = 0, t4B = 0, segs = 0, excl = 0;
+    bool u8A = true, u8B = true;
+    if (do_nb) {
+      const int seg8 = (cnt + 7) >> 3, seg4 = (cnt + 3) >> 2;
+      const int incl8 = wave_incl_scan(seg8);
+      const int incl4 = wave_incl_scan(seg4);
+      const int endA = halves ? kHalf - 1 : kWave - 1;
+      t8A = __builtin_amdgcn_readlane(incl8, endA);
+      t4A = __builtin_amdgcn_readlane(incl4, endA);
+      t8B = halves ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
+      t4B = halves ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
+      u8A = ((t8A + kWave - 1) / kWave) * (8 * kPairCost + kItemCost) <=
+            ((t4A + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+      u8B = ((t8B + kWave - 1) / kWave) * (8 * kPairCost + kItemCost) <=
+            ((t4B + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+      const bool use8 = h ? u8B : u8A;
+      segs = use8 ? seg8 : seg4;
+      excl = (use8 ? incl8 : incl4) - segs - (h ? (use8 ? t8A : t4A) : 0);
+      if (t8A + t8B > 0) {
+        ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
+        ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
+      }
+    }
     wave_sync();
     unsigned npair = 0;  // wave-wide count, credited to lane 0
-    if (use8)
-      pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot8, segs, excl, wgt, acc,
-                                                   npair, lane);
-    else
-      pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot4, segs, excl, wgt, acc,
-                                                   npair, lane);
+    for (int q = 0; q < nposes; ++q) {
+      if (q == 1) {  // A is complete; B's grid sums to lanes 0-31
+        uA = wave_sum_all(acc, lane);
+#pragma unroll
+        for (int i = 0; i < kAcc; ++i) acc[i] = h ? 0.f : ws->gridB[i][lane & (kHalf - 1)];
+      }
+      const bool u8 = q ? u8B : u8A;
+      const int tot = q ? (u8 ? t8B : t4B) : (u8 ? t8A : t4A);
+      const int sq = h == q ? segs : 0;  // the lanes of pose q mark its windows
+      const float wgt = q ? B.wgt : A.wgt;
+      if (tot > 0) {
+        if (u8)
+          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot, sq, excl, wgt, acc,
+                                                       npair, lane);
+        else
+          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot, sq, excl, wgt, acc,
+                                                       npair, lane);
+      }
+    }
     st[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
-  return wave_sum_all(acc, lane);
-}
-
-// Energy and dE/dM of pose p from the transposed sums: one lane per output word.
+  const float u = wave_sum_all(acc, lane);
+  if (nposes == 2)
+    uB = u;
+  else
+    uA = u;
+}// Energy and dE/dM of pose p from the transposed sums: one lane per output word.
 __device__ __forceinline__ void write_energrad(const KArgs& a, int p, float u, int lane) {
   if (lane & 3) return;
   const int i = acc_index(lane);
@@ -853,6 +914,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
       p = a.pose_order ? a.pose_order[k] : k;
       jid = a.pose_job ? a.pose_job[p] : 0;
       const AtjJob job = a.jobs[jid];
+      bs->lig[lane] = reinterpret_cast<const float4*>(job.ligand.beads);
+      bs->nb[lane] = job.ligand.nbeads;
       cnt = a.count[p];
       elast = a.elast[p];
       nst = max(0, min(a.nsteps_launch, job.nsteps - cnt));
@@ -877,18 +940,36 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
     for (int b = 0; b < nbatch; ++b) nst_max = max(nst_max, __builtin_amdgcn_readlane(nst, b));
     wave_sync();
     for (int s = 0; s < nst_max; ++s) {
+      // active slots in order; two consecutive ones of at most kHalf beads share the bead phase
       for (int b = 0; b < nbatch; ++b) {
         if (s >= __builtin_amdgcn_readlane(nst, b)) continue;
-        const AtjJob job = a.jobs[__builtin_amdgcn_readlane(jid, b)];
-        const float* m = bs->M[b];
-        const float M[16] = {m[0], m[1], m[2], 1.f, m[3], m[4], m[5], 1.f, m[6], m[7], m[8], 1.f,
-                             m[9], m[10], m[11], 1.f};
-        const float u =
-            pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws,
-                                                                      reinterpret_cast<const float4*>(job.ligand.beads),
-                                                                      job.ligand.nbeads, M, 0.f, st, lane);
+        PoseIn P[2];
+        int partner = -1;
+        P[0].lig = bs->lig[b];
+        P[0].nb = uni(bs->nb[b]);
+        P[0].M = bs->M[b];
+        P[0].wgt = 0.f;
+        P[1] = P[0];
+        P[1].nb = 0;
+        if (P[0].nb <= kHalf) {
+          int c = b + 1;
+          while (c < nbatch && s >= __builtin_amdgcn_readlane(nst, c)) ++c;
+          if (c < nbatch && uni(bs->nb[c]) <= kHalf) {
+            partner = c;
+            P[1].lig = bs->lig[c];
+            P[1].nb = uni(bs->nb[c]);
+            P[1].M = bs->M[c];
+          }
+        }
+        float uA, uB;
+        pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws, P[0], P[1],
+                                                                  st, lane, uA, uB);
         const int i = acc_index(lane);
-        if (!(lane & 3) && i < kAcc) bs->acc[b][i] = u;
+        if (!(lane & 3) && i < kAcc) {
+          bs->acc[b][i] = uA;
+          if (partner >= 0) bs->acc[partner][i] = uB;
+        }
+        if (partner >= 0) b = partner;
       }
       wave_sync();
       if (own && s < nst) {
@@ -966,12 +1047,26 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
       const int p = a.pose_order ? a.pose_order[k] : k;
       const int jid = a.pose_job ? a.pose_job[p] : 0;
       const AtjJob job = a.jobs[jid];
-      float M[16];
-      load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, M);
-      const float w = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      const float u = pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(
-          a, s_rec, s_par, s_side, s_pgrad, ws, reinterpret_cast<const float4*>(job.ligand.beads), job.ligand.nbeads,
-          M, w, st, lane);
+      float Mp[16];
+      load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, Mp);
+      wave_sync();  // the previous pose's readers of ws->M are done
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) ws->M[3 * k + l] = Mp[4 * k + l];
+      }
+      wave_sync();
+      PoseIn P[2];
+      P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
+      P[0].nb = job.ligand.nbeads;
+      P[0].M = ws->M;
+      P[0].wgt = (MODE == kAdjoint) ? a.weights[p] : 0.f;
+      P[1] = P[0];
+      P[1].nb = 0;
+      float u, unused;
+      pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, P[0], P[1], st,
+                                                           lane, u, unused);
       write_energrad(a, p, u, lane);
     }
   }
