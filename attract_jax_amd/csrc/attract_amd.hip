@@ -478,11 +478,13 @@ __host__ __device__ __forceinline__ int pair_slots(int nrt, int nlt) { return nr
 __host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots + 2) & ~1; }
 
 struct WaveScratch {
-  float4 xp[kWave];  // x', y', z', ligand ff type (int bits)
-  float4 xl[kWave];  // x, y, z (local), list start - S * first item (int bits)
-  int mark[kWave];   // bead whose first list segment is item base + i of the current window
-  float M[16];       // pose matrix of a single-pose evaluation (12 used: rotation rows, translation)
-  float gridB[kAcc][kHalf];  // grid sums of the second pose of a shared bead phase
+  float4 xp[kWave];          // x', y', z', ligand ff type (int bits)
+  float4 xl[kWave];          // x, y, z (local), list start - S * first item (int bits)
+  signed char mark[kWave];   // bead whose first list segment is item base + i of the current window
+  union {
+    float M[16];             // pose matrix of a single-pose evaluation (12 used: rotation rows, translation)
+    float4 gridB[kHalf];     // grid energy and gradient of the beads of the second pose of a shared bead phase
+  };
 };
 
 // Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
@@ -648,6 +650,18 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   }
 }
 
+// Adds one bead's grid energy E and gradient g (at local coordinates xl) to the lane
+// accumulators: energy, xl (x) g, g.
+__device__ __forceinline__ void bead_acc(float acc[kAcc], float E, const float g[3], const float xl[3]) {
+  acc[0] += E;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xl[k], g[l], acc[1 + 3 * k + l]);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+}
+
 // One pose of an evaluation: ligand beads, pose matrix in LDS (rows 0-2 of the rotation,
 // then the translation: 12 floats) and adjoint weight.
 struct PoseIn {
@@ -687,6 +701,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     const int nbh = h ? B.nb : A.nb;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
+    float Eg = 0.f, g[3] = {0.f, 0.f, 0.f}, raw[4];
     int lt = 0;
     if (j < nbh) {
       const float4 b = (h ? B.lig : A.lig)[j];
@@ -705,28 +720,18 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
         M[4 * k + 3] = 1.f;
       }
       transform(M, xl[0], xl[1], xl[2], xp);
-      float E, g[3], raw[4];
       int voxel;
       unsigned in_any;
-      bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, E, g, voxel, cnt, start, in_any, raw);
-      acc[0] += E;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xl[k], g[l], acc[1 + 3 * k + l]);
-#pragma unroll
-      for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+      bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
+      bead_acc(acc, Eg, g, xl);
       st[0] += in_any;
       if (voxel >= 0) {
         st[1] += 1;
         st[2] += (unsigned)cnt;
       }
     }
-    if (nposes == 2) {  // B's grid sums wait in LDS; its lanes start A's pair sums from zero
-      if (h) {
-#pragma unroll
-        for (int i = 0; i < kAcc; ++i) ws->gridB[i][lane - kHalf] = acc[i];
-      }
+    if (nposes == 2) {  // B's bead terms wait in LDS; its lanes start A's pair sums from zero
+      if (h) ws->gridB[lane - kHalf] = make_float4(Eg, g[0], g[1], g[2]);
 #pragma unroll
       for (int i = 0; i < kAcc; ++i) acc[i] = h ? 0.f : acc[i];
     }
@@ -749,18 +754,25 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const bool use8 = h ? u8B : u8A;
       segs = use8 ? seg8 : seg4;
       excl = (use8 ? incl8 : incl4) - segs - (h ? (use8 ? t8A : t4A) : 0);
-      if (t8A + t8B > 0) {
-        ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-        ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start - excl * (use8 ? 8 : 4)));
-      }
+      start -= excl * (use8 ? 8 : 4);
+    }
+    if (nposes == 2 || t8A + t8B > 0) {
+      ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
+      ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
     }
     wave_sync();
     unsigned npair = 0;  // wave-wide count, credited to lane 0
     for (int q = 0; q < nposes; ++q) {
-      if (q == 1) {  // A is complete; B's grid sums to lanes 0-31
+      if (q == 1) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
         uA = wave_sum_all(acc, lane);
 #pragma unroll
-        for (int i = 0; i < kAcc; ++i) acc[i] = h ? 0.f : ws->gridB[i][lane & (kHalf - 1)];
+        for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
+        if (!h) {
+          const float4 e = ws->gridB[lane];
+          const float4 x = ws->xl[kHalf + lane];
+          const float gb[3] = {e.y, e.z, e.w}, xb[3] = {x.x, x.y, x.z};
+          bead_acc(acc, e.x, gb, xb);
+        }
       }
       const bool u8 = q ? u8B : u8A;
       const int tot = q ? (u8 ? t8B : t4B) : (u8 ? t8A : t4A);
