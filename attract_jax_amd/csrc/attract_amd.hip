@@ -481,10 +481,7 @@ struct WaveScratch {
   float4 xp[kWave];          // x', y', z', ligand ff type (int bits)
   float4 xl[kWave];          // x, y, z (local), list start - S * first item (int bits)
   signed char mark[kWave];   // bead whose first list segment is item base + i of the current window
-  union {
-    float M[16];             // pose matrix of a single-pose evaluation (12 used: rotation rows, translation)
-    float4 gridB[kHalf];     // grid energy and gradient of the beads of the second pose of a shared bead phase
-  };
+  float4 gridB[kHalf];       // grid energy and gradient of the beads of the second pose of a shared bead phase
 };
 
 // Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
@@ -1034,13 +1031,15 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int ntp = a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
-  // wave scratch [nwaves] | minimizer batches [nwaves] or pair-table gradient (adjoint, [3][ntp])
+  // wave scratch [nwaves] | minimizer batches [nwaves], or pose matrices [nwaves][2][12] and
+  // (adjoint) the pair-table gradient [3][ntp]
   float4* s_par = smem;
   float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
   float4* s_rec = reinterpret_cast<float4*>(s_side + side_slots(nslots));
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
-  float* s_pgrad = MODE == kAdjoint ? reinterpret_cast<float*>(s_ws + nwaves) : nullptr;
+  float* s_mat = reinterpret_cast<float*>(s_ws + nwaves);
+  float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * 24 : nullptr;
   stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
@@ -1050,36 +1049,61 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
     minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, ws, bs, st, lane);
   } else {
+    // kBatch poses of the processing order per work-counter claim; two consecutive ones of at
+    // most kHalf beads share the bead phase
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
+    float* Ms = s_mat + wave * 24;
     for (;;) {
-      int k = 0;
-      if (lane == 0) k = atomicAdd(a.counter, 1);
-      k = __builtin_amdgcn_readfirstlane(k);
-      if (k >= a.N) break;
-      const int p = a.pose_order ? a.pose_order[k] : k;
-      const int jid = a.pose_job ? a.pose_job[p] : 0;
-      const AtjJob job = a.jobs[jid];
-      float Mp[16];
-      load_pose_matrix(a.poses + (size_t)p * (qt ? 7 : 16), qt, Mp);
-      wave_sync();  // the previous pose's readers of ws->M are done
-      if (lane == 0) {
+      int k0 = 0;
+      if (lane == 0) k0 = atomicAdd(a.counter, kBatch);
+      k0 = __builtin_amdgcn_readfirstlane(k0);
+      if (k0 >= a.N) break;
+      const int nbatch = min(kBatch, a.N - k0);
+      for (int b = 0; b < nbatch; ++b) {
+        int p[2];
+        PoseIn P[2];
+        p[0] = a.pose_order ? a.pose_order[k0 + b] : k0 + b;
+        const AtjJob job = a.jobs[a.pose_job ? a.pose_job[p[0]] : 0];
+        P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
+        P[0].nb = job.ligand.nbeads;
+        P[1] = P[0];
+        P[1].nb = 0;
+        p[1] = -1;
+        if (P[0].nb <= kHalf && b + 1 < nbatch) {
+          const int q = a.pose_order ? a.pose_order[k0 + b + 1] : k0 + b + 1;
+          const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[q] : 0];
+          if (job2.ligand.nbeads <= kHalf) {
+            p[1] = q;
+            P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
+            P[1].nb = job2.ligand.nbeads;
+          }
+        }
+        wave_sync();  // the previous evaluation's readers of Ms are done
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int h = 0; h < 2; ++h) {
+          P[h].M = Ms + 12 * h;
+          P[h].wgt = 0.f;
+          if (h == 1 && p[1] < 0) continue;
+          float Mp[16];
+          load_pose_matrix(a.poses + (size_t)p[h] * (qt ? 7 : 16), qt, Mp);
+          if (MODE == kAdjoint) P[h].wgt = a.weights[p[h]];
+          if (lane == 0) {
 #pragma unroll
-          for (int l = 0; l < 3; ++l) ws->M[3 * k + l] = Mp[4 * k + l];
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+              for (int l = 0; l < 3; ++l) Ms[12 * h + 3 * k + l] = Mp[4 * k + l];
+          }
+        }
+        wave_sync();
+        float u[2];
+        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, P[0], P[1], st,
+                                                             lane, u[0], u[1]);
+        write_energrad(a, p[0], u[0], lane);
+        if (p[1] >= 0) {
+          write_energrad(a, p[1], u[1], lane);
+          ++b;
+        }
       }
-      wave_sync();
-      PoseIn P[2];
-      P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
-      P[0].nb = job.ligand.nbeads;
-      P[0].M = ws->M;
-      P[0].wgt = (MODE == kAdjoint) ? a.weights[p] : 0.f;
-      P[1] = P[0];
-      P[1].nb = 0;
-      float u, unused;
-      pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, P[0], P[1], st,
-                                                           lane, u, unused);
-      write_energrad(a, p, u, lane);
     }
   }
   if (a.stats) {
@@ -1540,7 +1564,7 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   const size_t ntp = (size_t)a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
-                      wpb * sizeof(WaveScratch) + (MODE == kMinimize ? wpb * sizeof(BatchScratch) : 0) +
+                      wpb * sizeof(WaveScratch) + (MODE == kMinimize ? wpb * sizeof(BatchScratch) : wpb * 24 * 4) +
                       (MODE == kAdjoint ? 3 * ntp * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
