@@ -479,7 +479,7 @@ __host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots 
 
 struct WaveScratch {
   float4 xp[kWave];          // x', y', z', ligand ff type (int bits)
-  float4 xl[kWave];          // x, y, z (local), list start - S * first item (int bits)
+  float4 xl[kWave];          // -2 (x, y, z) (local), list start - S * first item (int bits)
   signed char mark[kWave];   // bead whose first list segment is item base + i of the current window
   float4 gridB[kHalf];       // grid energy and gradient of the beads of the second pose of a shared bead phase
 };
@@ -635,15 +635,17 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         }
       }
     }
+    // the gradient is -2 (gx, gy, gz); bl holds -2 x_local, and scaling by -2 is exact, so
+    // fma(-2 x, g', acc) and fma(-2, g', acc) equal the products with the scaled gradient
     acc[0] += e;
-    const float g[3] = {-2.f * gx, -2.f * gy, -2.f * gz};
+    const float g[3] = {gx, gy, gz};
     const float xb[3] = {bl.x, bl.y, bl.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
 #pragma unroll
-    for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
+    for (int l = 0; l < 3; ++l) acc[10 + l] = fmaf(-2.f, g[l], acc[10 + l]);
   }
 }
 
@@ -755,7 +757,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     if (nposes == 2 || t8A + t8B > 0) {
       ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-      ws->xl[lane] = make_float4(xl[0], xl[1], xl[2], __int_as_float(start));
+      ws->xl[lane] = make_float4(-2.f * xl[0], -2.f * xl[1], -2.f * xl[2], __int_as_float(start));
     }
     wave_sync();
     unsigned npair = 0;  // wave-wide count, credited to lane 0
@@ -766,8 +768,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
         if (!h) {
           const float4 e = ws->gridB[lane];
-          const float4 x = ws->xl[kHalf + lane];
-          const float gb[3] = {e.y, e.z, e.w}, xb[3] = {x.x, x.y, x.z};
+          const float4 x = ws->xl[kHalf + lane];  // -2 x_local
+          const float gb[3] = {e.y, e.z, e.w}, xb[3] = {-0.5f * x.x, -0.5f * x.y, -0.5f * x.z};
           bead_acc(acc, e.x, gb, xb);
         }
       }
