@@ -41,6 +41,45 @@ def _counter_per_dispatch(csv_path, counter):
     return out
 
 
+def scoring_rates(dev, wpb):
+    """Side figures beside the minimizer: one energrad of the C2 workload (131,072 poses of a
+    21-bead ligand against the C2 grid) and one pair-table adjoint of the C4 workload (65,536
+    poses), each the best of three launches timed with HIP events on the launch stream."""
+    import torch
+
+    from attract_jax_amd import _lib
+    from attract_jax_amd.autodiff import param_adjoint
+    from attract_jax_amd.pose import mat_from_qt
+    from attract_jax_amd.workloads import build_system, c2, c4
+
+    def best_ms(fn, stream):
+        fn()
+        ms = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return min(ms)
+
+    stream = torch.cuda.current_stream(dev)
+    w2 = c2()
+    system, ligs = build_system(w2, dev, launch=_lib.AtjLaunch(wpb, 0, 0))
+    mat = torch.from_numpy(mat_from_qt(w2.quat, w2.trans).astype(np.float32)).to(dev)
+    ms2 = best_ms(lambda: system.energrad(mat, ligs[0]), stream)
+    w4 = c4()
+    m4 = torch.from_numpy(mat_from_qt(w4.quat, w4.trans).astype(np.float32)).to(dev).reshape(-1, 16)
+    wts = torch.ones(m4.shape[0], device=dev)
+    ms4 = best_ms(lambda: param_adjoint(system, ligs[0], m4, system.ff, wts, system.flags), stream)
+    log(f"[bench] scoring: C2 energrad {ms2:.3f} ms, C4 adjoint {ms4:.3f} ms")
+    return {"c2_energrad_poses_per_s": len(w2.quat) / (ms2 * 1e-3), "c2_energrad_ms": ms2, "c2_poses": len(w2.quat),
+            "c4_adjoint_poses_per_s": len(w4.quat) / (ms4 * 1e-3), "c4_adjoint_ms": ms4, "c4_poses": len(w4.quat),
+            "note": "energies + dE/dM of every pose (C2); sum over poses of dE/d(rc, ac, emin) with its partial-sum "
+                    "reduction (C4); best of 3 launches"}
+
+
 def measure_traffic(args, wpb, opt):
     """HBM traffic of one minimizer launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
     timed pose-kernel dispatch of a one-launch child run of this workload, one rocprofv3 pass
@@ -55,7 +94,7 @@ def measure_traffic(args, wpb, opt):
     if exe is None:
         return None
     child = [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-             "--no-plain", "--serve-jobs", "0", "--no-traffic", "--copy-calibration", "--waves", str(wpb),
+             "--no-plain", "--serve-jobs", "0", "--no-traffic", "--no-scoring", "--copy-calibration", "--waves", str(wpb),
              "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
              str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps), "--nshell", str(args.nshell)]
     env = dict(os.environ, TMPDIR="/tmp")
@@ -111,6 +150,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean job arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-scoring", action="store_true",
+                    help="skip the single-shot energrad (C2) and pair-table adjoint (C4) timings")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the minimizer's HBM traffic")
     args = ap.parse_args()
@@ -285,6 +326,9 @@ def main():
             "sample": f"FP32 C/OpenMP twin of the same algorithm: {per_lig} poses of each of the {len(twins)} "
                       f"ligands x {w.nsteps} ADAM steps ({tc:.1f}s), {cpu_model}"}
         log(f"[bench] cpu twin: {cps / tc:.3e} pose-steps/s on {ncores} threads over {tc:.1f}s")
+
+    if not args.no_scoring:
+        result["scoring"] = scoring_rates(dev, system.launch.waves_per_block)
 
     if args.serve_jobs:
         from attract_jax_amd import workloads as W
