@@ -753,8 +753,9 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const bool use8 = h ? u8B : u8A;
       segs = use8 ? seg8 : seg4;
       excl = (use8 ? incl8 : incl4) - segs - (h ? (use8 ? t8A : t4A) : 0);
-      start -= excl * (use8 ? 8 : 4);
     }
+    const int lstart = start;  // the bead's list start
+    start -= excl * (h ? (u8B ? 8 : 4) : (u8A ? 8 : 4));
     if (nposes == 2 || t8A + t8B > 0) {
       ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
       ws->xl[lane] = make_float4(-2.f * xl[0], -2.f * xl[1], -2.f * xl[2], __int_as_float(start));
@@ -777,14 +778,31 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const int tot = q ? (u8 ? t8B : t4B) : (u8 ? t8A : t4A);
       const int sq = h == q ? segs : 0;  // the lanes of pose q mark its windows
       const float wgt = q ? B.wgt : A.wgt;
-      if (tot > 0) {
-        if (u8)
-          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot, sq, excl, wgt, acc,
-                                                       npair, lane);
-        else
-          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, tot, sq, excl, wgt, acc,
-                                                       npair, lane);
+      // 8-entry segments: full windows, and a last window at most half full re-cut when the
+      // entries it would hold (the rest of each bead's list) fit one window of 4-entry segments
+      bool tail4 = false;
+      int seg4t = 0, excl4t = 0, tot4t = 0, done = 0;
+      const int nfull = tot & ~(kWave - 1);
+      if (u8 && nfull > 0 && tot - nfull <= kWave / 2) {
+        done = 8 * min(max(nfull - excl, 0), sq);  // entries in full windows (sq = 0: other pose)
+        seg4t = h == q ? (max(cnt - done, 0) + 3) >> 2 : 0;
+        const int incl = wave_incl_scan(seg4t);
+        tot4t = __builtin_amdgcn_readlane(incl, kWave - 1);
+        excl4t = incl - seg4t;
+        tail4 = tot4t <= kWave;
       }
+      const int n8 = u8 ? (tail4 ? nfull : tot) : 0;
+      const int n4 = u8 ? (tail4 ? tot4t : 0) : tot;
+      if (n8 > 0)
+        pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt, acc,
+                                                     npair, lane);
+      if (tail4) {
+        if (h == q) ws->xl[lane].w = __int_as_float(lstart + done - 4 * excl4t);
+        wave_sync();
+      }
+      if (n4 > 0)
+        pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, n4, tail4 ? seg4t : sq,
+                                                     tail4 ? excl4t : excl, wgt, acc, npair, lane);
     }
     st[3] += lane == 0 ? npair : 0u;
     wave_sync();
