@@ -661,6 +661,14 @@ __device__ __forceinline__ void bead_acc(float acc[kAcc], float E, const float g
   for (int l = 0; l < 3; ++l) acc[10 + l] += g[l];
 }
 
+// Estimated cost of t 8-entry segments: full windows, and a last window at most half full
+// taken as one window of 4-entry segments (the re-cut of the pair phase).
+__device__ __forceinline__ int seg8_cost(int t) {
+  const int full = t / kWave, rest = t - full * kWave;
+  return full * (8 * kPairCost + kItemCost) +
+         (rest == 0 ? 0 : (full > 0 && rest <= kWave / 2 ? 4 : 8) * kPairCost + kItemCost);
+}
+
 // One pose of an evaluation: ligand beads, pose matrix in LDS (rows 0-2 of the rotation,
 // then the translation: 12 floats) and adjoint weight.
 struct PoseIn {
@@ -746,10 +754,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       t4A = __builtin_amdgcn_readlane(incl4, endA);
       t8B = halves ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
       t4B = halves ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
-      u8A = ((t8A + kWave - 1) / kWave) * (8 * kPairCost + kItemCost) <=
-            ((t4A + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
-      u8B = ((t8B + kWave - 1) / kWave) * (8 * kPairCost + kItemCost) <=
-            ((t4B + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+      u8A = seg8_cost(t8A) <= ((t4A + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
+      u8B = seg8_cost(t8B) <= ((t4B + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
       const bool use8 = h ? u8B : u8A;
       segs = use8 ? seg8 : seg4;
       excl = (use8 ? incl8 : incl4) - segs - (h ? (use8 ? t8A : t4A) : 0);
