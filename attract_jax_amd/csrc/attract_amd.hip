@@ -492,7 +492,7 @@ struct BatchScratch {
   const float4* lig[kBatch];  // ligand beads of the slot's job
   int nb[kBatch];             // and their number
   float M[kBatch][12];    // rotation rows and translation for the next evaluation
-  float acc[kBatch][16];  // reduced sums of the last evaluation (kAcc used)
+  float acc[kBatch][16];  // reduced sums of the last evaluation (kAcc), then c1, c2 and the last energy
   float prm[kBatch][7];   // quaternion (x, y, z, w), translation
   float m1[kBatch][7];    // ADAM moments
   float m2[kBatch][7];
@@ -873,6 +873,41 @@ __device__ __forceinline__ float optimizer_update(const AtjOptimizer& o, float p
 }
 
 // Rotation rows and translation (12 floats) of a (quaternion, translation) pose.
+// Parameter i of one pose's optimizer update: the operations optimizer_update performs for
+// it, before the quaternion renormalisation (quat_renorm).  prm and acc are the pose's
+// parameters and reduced sums; returns the new parameter and writes its moments and the
+// energy including the restraint.
+__device__ __forceinline__ float optimizer_update_param(const AtjOptimizer& o, const float* prm, const float* acc,
+                                                        int i, float m1_old, float m2_old, float c1, float c2,
+                                                        float& m1, float& m2, float& E) {
+  const float q[4] = {prm[0], prm[1], prm[2], prm[3]};
+  const float A[9] = {acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7], acc[8], acc[9]};
+  float gq[4];
+  quat_grad(q, A, gq);
+  float g = i == 0 ? gq[0] : i == 1 ? gq[1] : i == 2 ? gq[2] : i == 3 ? gq[3] : acc[10 + (i > 3 ? i - 4 : 0)];
+  const float x = prm[i];
+  E = acc[0];
+  if (o.gravity != 0.f) {
+    const float n = sqrtf(fmaf(prm[6], prm[6], fmaf(prm[5], prm[5], prm[4] * prm[4])));
+    E = E + (n * n) * o.gravity;
+    if (i >= 4) g = g + (2.f * o.gravity) * x;
+  }
+  const float b1 = o.beta1, b2 = o.beta2;
+  const bool belief = o.kind == ATJ_OPT_ADABELIEF;
+  const float pe = belief ? g - m1_old : g;
+  m1 = (1.f - b1) * g + b1 * m1_old;
+  m2 = (1.f - b2) * (pe * pe) + b2 * m2_old;
+  if (belief) m2 = m2 + o.eps_root;
+  const float upd = (m1 / c1) / (sqrtf(m2 / c2) + o.eps);
+  return x + (-(i < 4 ? o.lr_rot : o.lr_trans)) * upd;
+}
+
+// Component i (< 4) of the renormalised quaternion prm[0..3], as optimizer_update computes it.
+__device__ __forceinline__ float quat_renorm(const float* prm, int i) {
+  const float nrm = sqrtf(fmaf(prm[3], prm[3], fmaf(prm[2], prm[2], fmaf(prm[1], prm[1], prm[0] * prm[0]))));
+  return prm[i] / nrm;
+}
+
 __device__ __forceinline__ void pose12(const float prm[7], float M[12]) {
   float R[9];
   quat_to_mat(prm, R);
@@ -881,6 +916,19 @@ __device__ __forceinline__ void pose12(const float prm[7], float M[12]) {
   M[9] = prm[4];
   M[10] = prm[5];
   M[11] = prm[6];
+}
+
+// Elements of pose12(prm) written by lane i (< 7) of a pose's update: M[i], and M[7 + i]
+// for i < 5 (rotation rows, then the translation).
+__device__ __forceinline__ void pose12_elements(const float* prm, float* M, int i) {
+  const float q[4] = {prm[0], prm[1], prm[2], prm[3]};
+  float R[9];
+  quat_to_mat(q, R);
+  float lo = R[0];
+#pragma unroll
+  for (int k = 1; k < 7; ++k) lo = i == k ? R[k] : lo;
+  M[i] = lo;
+  if (i < 5) M[7 + i] = i == 0 ? R[7] : i == 1 ? R[8] : prm[2 + i];
 }
 
 __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float M[16]) {
@@ -935,6 +983,8 @@ template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               const float2* __restrict__ s_side, WaveScratch* ws, BatchScratch* bs, unsigned st[4],
                               int lane) {
+  // Slot lanes of the optimizer update: lane 8 b + i updates parameter i (< 7) of slot b.
+  const int sb = lane >> 3, pi = lane & 7;
   for (;;) {
     int k0 = 0;
     if (lane == 0) k0 = atomicAdd(a.counter, kBatch);
@@ -943,7 +993,6 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
     const int nbatch = min(kBatch, a.N - k0);
     const bool own = lane < nbatch;
     int p = 0, jid = 0, cnt = 0, nst = 0;
-    float c1 = 0.f, c2 = 0.f, elast = 0.f;
     if (own) {
       const int k = k0 + lane;
       p = a.pose_order ? a.pose_order[k] : k;
@@ -952,13 +1001,17 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
       bs->lig[lane] = reinterpret_cast<const float4*>(job.ligand.beads);
       bs->nb[lane] = job.ligand.nbeads;
       cnt = a.count[p];
-      elast = a.elast[p];
       nst = max(0, min(a.nsteps_launch, job.nsteps - cnt));
-      // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0
+      // bias corrections 1 - b^t by the recurrence c_t = b c_{t-1} + (1 - b), c_0 = 0, and the
+      // last energy, kept in the unused sums 13-15 of the slot's row
+      float c1 = 0.f, c2 = 0.f;
       for (int i = 0; i < cnt; ++i) {
         c1 = fmaf(job.opt.beta1, c1, 1.f - job.opt.beta1);
         c2 = fmaf(job.opt.beta2, c2, 1.f - job.opt.beta2);
       }
+      bs->acc[lane][kAcc] = c1;
+      bs->acc[lane][kAcc + 1] = c2;
+      bs->acc[lane][kAcc + 2] = a.elast[p];
       float prm[7], M[12];
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
@@ -973,6 +1026,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     int nst_max = 0;
     for (int b = 0; b < nbatch; ++b) nst_max = max(nst_max, __builtin_amdgcn_readlane(nst, b));
+    const int src = sb < nbatch ? sb : 0;  // the own lane of this lane's slot
+    const bool slot = sb < nbatch && pi < 7;
     wave_sync();
     for (int s = 0; s < nst_max; ++s) {
       // active slots in order; two consecutive ones of at most kHalf beads share the bead phase
@@ -1007,31 +1062,34 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
         if (partner >= 0) b = partner;
       }
       wave_sync();
-      if (own && s < nst) {
-        const AtjJob job = a.jobs[jid];
-        float prm[7], m1[7], m2[7], acc[kAcc], M[12];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          prm[i] = bs->prm[lane][i];
-          m1[i] = bs->m1[lane][i];
-          m2[i] = bs->m2[lane][i];
-        }
-#pragma unroll
-        for (int i = 0; i < kAcc; ++i) acc[i] = bs->acc[lane][i];
-        ++cnt;
-        c1 = fmaf(job.opt.beta1, c1, 1.f - job.opt.beta1);
-        c2 = fmaf(job.opt.beta2, c2, 1.f - job.opt.beta2);
-        elast = optimizer_update(job.opt, prm, m1, m2, acc, c1, c2);
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          bs->prm[lane][i] = prm[i];
-          bs->m1[lane][i] = m1[i];
-          bs->m2[lane][i] = m2[i];
-        }
-        pose12(prm, M);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) bs->M[lane][i] = M[i];
+      // optimizer update, one lane per (slot, parameter): the operations of optimizer_update
+      const int nst_s = __shfl(nst, src), jid_s = __shfl(jid, src);
+      const bool act = slot && s < nst_s;
+      float x = 0.f, m1 = 0.f, m2 = 0.f, c1 = 0.f, c2 = 0.f, E = 0.f;
+      if (act) {
+        const AtjOptimizer o = a.jobs[jid_s].opt;
+        c1 = fmaf(o.beta1, bs->acc[sb][kAcc], 1.f - o.beta1);
+        c2 = fmaf(o.beta2, bs->acc[sb][kAcc + 1], 1.f - o.beta2);
+        x = optimizer_update_param(o, bs->prm[sb], bs->acc[sb], pi, bs->m1[sb][pi], bs->m2[sb][pi], c1, c2,
+                                   m1, m2, E);
       }
+      wave_sync();
+      if (act) {
+        bs->prm[sb][pi] = x;
+        bs->m1[sb][pi] = m1;
+        bs->m2[sb][pi] = m2;
+        if (pi == 0) {
+          bs->acc[sb][kAcc] = c1;
+          bs->acc[sb][kAcc + 1] = c2;
+          bs->acc[sb][kAcc + 2] = E;
+        }
+      }
+      wave_sync();
+      if (act && pi < 4) x = quat_renorm(bs->prm[sb], pi);
+      wave_sync();
+      if (act && pi < 4) bs->prm[sb][pi] = x;
+      wave_sync();
+      if (act) pose12_elements(bs->prm[sb], bs->M[sb], pi);
       wave_sync();
     }
     if (own) {
@@ -1041,8 +1099,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
         a.mom1[7 * p + i] = bs->m1[lane][i];
         a.mom2[7 * p + i] = bs->m2[lane][i];
       }
-      a.count[p] = cnt;
-      a.elast[p] = elast;
+      a.count[p] = cnt + nst;
+      a.elast[p] = bs->acc[lane][kAcc + 2];
     }
     wave_sync();
   }
