@@ -818,7 +818,9 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     uB = u;
   else
     uA = u;
-}// Energy and dE/dM of pose p from the transposed sums: one lane per output word.
+}
+
+// Energy and dE/dM of pose p from the transposed sums: one lane per output word.
 __device__ __forceinline__ void write_energrad(const KArgs& a, int p, float u, int lane) {
   if (lane & 3) return;
   const int i = acc_index(lane);
