@@ -1502,7 +1502,6 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
     for (int i = 0; i < 3; ++i)
       if (g->dim2[i] < 2) return fail(ATJ_ERR_INVALID, "outer grid dim2 must be >= 2");
   if (need_lists && (!g->nbgrid || !g->nblist)) return fail(ATJ_ERR_INVALID, "neighbour lists are NULL");
-  if (need_lists && g->nshell > 0 && !g->nbshell) return fail(ATJ_ERR_INVALID, "nbshell is NULL");
   G.ox = g->origin[0];
   G.oy = g->origin[1];
   G.oz = g->origin[2];
@@ -1519,6 +1518,7 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.ntypes = g->ntypes;
   G.layout = g->layout;
   if (g->nshell < 0 || g->nshell > kMaxShell) return fail(ATJ_ERR_INVALID, "nshell must be 0..6");
+  if (need_lists && g->nshell > 0 && !g->nbshell) return fail(ATJ_ERR_INVALID, "nbshell is NULL");
   G.nshell = g->nshell;
   G.shinv = (float)(2.0 * g->nshell / std::sqrt(3.0));
   G.lx = (float)(g->dim[0] - 1);
@@ -1609,7 +1609,7 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
     if (per_cu < 1) per_cu = 1;
     blocks = per_cu * d.cus;
     // poses a block takes per work-counter fetch round
-    const long long per_block = (long long)wpb * (MODE == kMinimize ? kBatch : 1);
+    const long long per_block = (long long)wpb * kBatch;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
