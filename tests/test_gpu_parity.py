@@ -322,3 +322,35 @@ def test_batch_straddles_jobs_with_different_step_counts(s1):
         alone = PoseState.create(q[lo:hi], t[lo:hi], DEV)
         minimize(s1.system, alone_plan, alone, steps[k])
         assert torch.equal(alone.pose, st.pose[lo:hi]) and torch.equal(alone.energy, st.energy[lo:hi])
+
+
+def test_ligand_size_boundaries_shared_bead_phase_and_chunks():
+    """Ligands of 31-33 beads (two poses of at most 32 share a bead phase) and 63-65 beads
+    (one or two chunks of 64): minimizer and energrad results equal those of each ligand
+    alone, bit for bit, and the CPU twin within 1e-4."""
+    from attract_jax_amd.workloads import LigandSpec, random_chain
+    sizes = [31, 32, 33, 63, 64, 65]
+    w = c3(nligands=len(sizes), poses_per_ligand=40, nsteps=15)
+    rng = np.random.default_rng(11)
+    for k, n in enumerate(sizes):
+        w.ligands[k] = LigandSpec(random_chain(n, rng), rng.integers(1, 33, n))
+    system, ligs = build_system(w, DEV, keep_plain=True)
+    grid = system.grid.to_reference_grid()
+    plan = BatchPlan(system, ligs, w.pose_counts, w.nsteps, w.lr_rot, w.lr_trans)
+    st = PoseState.create(w.quat, w.trans, DEV)
+    minimize(system, plan, st, w.nsteps)
+    for k in range(len(ligs)):
+        sl = slice(plan.offsets[k], plan.offsets[k + 1])
+        alone_plan = BatchPlan(system, [ligs[k]], [w.pose_counts[k]], w.nsteps, w.lr_rot, w.lr_trans)
+        alone = PoseState.create(w.quat[sl], w.trans[sl], DEV)
+        minimize(system, alone_plan, alone, w.nsteps)
+        assert torch.equal(alone.pose, st.pose[sl]) and torch.equal(alone.energy, st.energy[sl]), sizes[k]
+        l = w.ligands[k]
+        tw = Twin(grid, _ff_arrays(system), w.coor_rec, w.rec_types - 1, l.coor, l.types - 1, l.types - 1)
+        mat = mat_from_qt(w.quat[sl], w.trans[sl]).astype(np.float32)
+        E, G = system.energrad(mat, ligs[k])
+        E1, G1 = system.energrad(mat[:1], ligs[k])  # alone in its launch: no bead-phase partner
+        assert torch.equal(E[:1], E1) and torch.equal(G[:1], G1), sizes[k]
+        Et, Gt = tw.energrad(mat)
+        assert _rel(E.cpu().numpy(), Et).max() < 1e-4, sizes[k]
+        assert _grel(G.cpu().numpy(), Gt).max() < 1e-4, sizes[k]
