@@ -354,3 +354,43 @@ def test_ligand_size_boundaries_shared_bead_phase_and_chunks():
         Et, Gt = tw.energrad(mat)
         assert _rel(E.cpu().numpy(), Et).max() < 1e-4, sizes[k]
         assert _grel(G.cpu().numpy(), Gt).max() < 1e-4, sizes[k]
+
+
+def test_neighbour_only_grid_without_plateau(s1):
+    """A grid without potentials (test-nb-energrad.py: lists only, plain nonbon without the
+    plateau) through the pose kernels: energrad and minimization vs the CPU twin (T_NB) and
+    the FP64 oracle, in whole-list and shell order."""
+    from attract_jax_amd.device import DeviceGrid
+    from attract_jax_amd.score import ScoringSystem
+    w = s1.w
+    rec = s1.system.receptor
+    dg = DeviceGrid.build(rec, s1.system.ff, np.arange(32), potentials=False, device=DEV, **w.grid_kw)
+    assert not dg.has_potentials
+    lig = w.ligands[0]
+    mat = s1.mats(slice(0, 300))
+    tw = None
+    for nshell in (0, 6):
+        system = ScoringSystem(DeviceGrid.build(rec, s1.system.ff, np.arange(32), potentials=False, device=DEV,
+                                                **w.grid_kw), s1.system.ff, rec, nshell=nshell)
+        assert system.flags == _lib.ATJ_TERM_NB
+        E, G = system.energrad(mat, s1.ligs[0])
+        E, G = E.cpu().numpy(), G.cpu().numpy()
+        if tw is None:
+            gref = dg.to_reference_grid()
+            tw = Twin(gref, _ff_arrays(s1.system), w.coor_rec, w.rec_types - 1, lig.coor, lig.types - 1,
+                      lig.types - 1)
+            Eo, Go = O.energrad(mat.astype(np.float64), lig.coor, lig.types - 1, lig.types - 1, w.coor_rec,
+                                w.rec_types - 1, s1.ff, gref, terms=("nb",), plateau=False)
+        Et, Gt = tw.energrad(mat, flags=T_NB)
+        assert _rel(E, Et).max() < 1e-4 and _grel(G, Gt).max() < 1e-4, nshell
+        assert _rel(E, Eo).max() < 1e-4 and _grel(G, Go).max() < 1e-4, nshell
+        assert np.count_nonzero(E) > 0
+        plan = BatchPlan(system, s1.ligs, [100], 10, w.lr_rot, w.lr_trans)
+        st = PoseState.create(w.quat[:100], w.trans[:100], DEV)
+        minimize(system, plan, st, 10)
+        Pt, _ = tw.minimize(np.concatenate([w.quat[:100], w.trans[:100]], 1), 10, w.lr_rot, w.lr_trans, flags=T_NB)
+        P = st.pose.cpu().numpy()
+        xg = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), lig.coor)
+        xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
+        r = O.rmsd(xg, xt)
+        assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95, nshell
