@@ -394,3 +394,27 @@ def test_neighbour_only_grid_without_plateau(s1):
         xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
         r = O.rmsd(xg, xt)
         assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95, nshell
+
+
+def test_grid_term_only(s1):
+    """Flags with the grid term alone (potential_energy of test-grid-energrad.py:241-244 as a
+    pose energy): energrad and minimization vs the CPU twin (T_GRID), including the shared
+    bead phase of the 4-bead ligand."""
+    w = s1.w
+    flags = _lib.ATJ_TERM_GRID
+    mat = s1.mats(slice(0, 300))
+    E, G = s1.system.energrad(mat, s1.ligs[0], flags=flags)
+    Et, Gt = s1.twins[0].energrad(mat, flags=T_GRID)
+    assert _rel(E.cpu().numpy(), Et).max() < 1e-4 and _grel(G.cpu().numpy(), Gt).max() < 1e-4
+    assert np.count_nonzero(Et) > 0
+    plan = BatchPlan(s1.system, s1.ligs, [100], 10, w.lr_rot, w.lr_trans)
+    st = PoseState.create(w.quat[:100], w.trans[:100], DEV)
+    minimize(s1.system, plan, st, 10, flags=flags)
+    Pt, _ = s1.twins[0].minimize(np.concatenate([w.quat[:100], w.trans[:100]], 1), 10, w.lr_rot, w.lr_trans,
+                                 flags=T_GRID)
+    lig = w.ligands[0]
+    P = st.pose.cpu().numpy()
+    xg = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), lig.coor)
+    xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
+    r = O.rmsd(xg, xt)
+    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
