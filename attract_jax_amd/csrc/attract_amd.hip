@@ -1,19 +1,22 @@
 // libattract_amd: ATTRACT rigid-body docking scoring and minimization for gfx950.
 //
-// One wavefront (64 lanes) owns one pose at a time and pulls poses from a
-// device work counter until the list is exhausted (persistent kernel).  For
+// One wavefront (64 lanes) evaluates one pose at a time and claims poses from a
+// device work counter, 8 per claim, until the list is exhausted (persistent kernel).  For
 // each chunk of up to 64 ligand beads:
 //   * bead phase  -- lane = bead: rigid transform in registers, grid trilinear
 //     term (one 128-byte record per bead in the corner-packed layout), voxel of
 //     the neighbour lookup, (count, start) of its receptor list;
 //   * pair phase  -- the wave walks the concatenation of all the chunk's lists
-//     in 16-byte segments of 8 receptor indices (lane = segment), so lanes stay
+//     in segments of 8 receptor indices (4 in a short last window; lane =
+//     segment), so lanes stay
 //     busy whatever the per-bead list lengths; receptor beads and the pair table
 //     are read from LDS.
 // Energy and dE/dM (13 floats) are accumulated per lane and reduced together by a
-// transposed butterfly of lane swaps, in a fixed order.  The minimizer gives each
-// wavefront a batch of poses: per ADAM step it evaluates them one after the other,
-// then updates them together, one lane per pose, with the state in LDS.
+// transposed butterfly of lane swaps, in a fixed order.  Two consecutive poses of at
+// most 32 beads share one bead phase (lanes 0-31 and 32-63), each pose's sums still
+// formed exactly as when it is evaluated alone.  The minimizer gives each wavefront a
+// batch of 8 poses: per optimizer step it evaluates them in order, then updates them
+// together, one lane per (pose, parameter), with the state in LDS.
 //
 // The file is compiled with -ffp-contract=off: every fused multiply-add below
 // is written explicitly, so the CPU twin can pin the operation order of the
@@ -485,8 +488,8 @@ struct WaveScratch {
 };
 
 // Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
-// evaluates one after the other, and whose ADAM updates it then runs together (lane =
-// slot).
+// evaluates in order (two small ones at a time where they share a bead phase), and whose
+// optimizer updates it then runs together (lane 8 slot + parameter).
 constexpr int kBatch = 8;
 struct BatchScratch {
   const float4* lig[kBatch];  // ligand beads of the slot's job
@@ -978,8 +981,9 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
 }
 
 // Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
-// processing order at a time; per ADAM step it evaluates them one after the other (all
-// 64 lanes on one pose), then lane b runs the ADAM update of slot b.  Slots may belong to
+// processing order at a time; per optimizer step it evaluates the slots still stepping
+// in order (all 64 lanes on one pose, or a slot of at most kHalf beads together with the
+// next such slot), then lane 8 b + i updates parameter i of slot b.  Slots may belong to
 // different jobs and have different step counts.
 template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
