@@ -23,6 +23,7 @@
 // transform and the voxelisation bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -380,15 +381,19 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, ivor * rmin2, c_lo) and 8-byte side record (c_hi, gh) of
-// one pair type.  ivor = +-1 rides on the sign of rmin2 (ATTRACT's ipon is +-1,
-// attract-original/convert-attract-par.py:14-15); an ivor of another magnitude poisons
-// the record (NaN energies) rather than being misread.  The pair energy is
-// s vlj + (dsq < rmin2 ? c_lo : c_hi) and its gradient factor -2 (s dh + gh / sqrt(dsq))
-// (see pair_eval), so everything that depends on the pair type only is folded in here:
-// c_lo = (ivor - 1) emin and c_hi = 0 without the plateau; with it both carry
-// -E(plateaudis) and gh = P dE/ddsq (P), the force-shift factor -2 P dE/ddsq (P)
-// (test-grid-energrad.py:124-147) divided by -2; zero for the exact derivative.
+// 16-byte pair record (rc, ac, r2, hw) and 8-byte side record (k, gh) of one pair type.
+// ATTRACT's pair energy (test-grid-energrad.py:95-111) is vlj + (ivor - 1) emin for
+// dsq < rmin2 and ivor vlj beyond, ivor = +-1 (ipon, attract-original/
+// convert-attract-par.py:14-15); an ivor of another magnitude poisons the record (NaN
+// energies) rather than being misread.  With w = (ivor - 1) emin, lo = dsq < rmin2 and
+// s = lo ? 1 : ivor, both branches are s (vlj + w / 2) + w / 2: the record holds hw = w / 2
+// and k = w / 2 (minus E(plateaudis) with the plateau), so that E = s (vlj + hw) + k needs
+// no per-pair choice of constants.  For ivor = +1, w = 0 and s = 1 on both sides of rmin2,
+// so the record stores r2 = FLT_MAX and lo always holds; for ivor = -1 it stores
+// r2 = rmin2 (infinity for rmin2 >= FLT_MAX), and s = lo ? 1 : -1 either way.  The
+// gradient factor is -2 (s dh + gh / sqrt(dsq)) (see pair_eval), gh = P dE/ddsq (P) being
+// the force-shift factor -2 P dE/ddsq (P) (test-grid-energrad.py:124-147) divided by -2;
+// zero for the exact derivative and without the plateau.
 struct PairType {
   float4 rec;
   float2 side;
@@ -398,16 +403,16 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
                                               bool exact, const PlateauConst& pc) {
   if (fabsf(iv) != 1.f) rc = __int_as_float(0x7fc00000);
   const float w = (iv - 1.f) * em;
+  const float hw = 0.5f * w;
   PairType t;
-  t.rec = make_float4(rc, ac, iv < 0.f ? -r2 : r2, w);
-  t.side = make_float2(0.f, 0.f);
+  t.rec = make_float4(rc, ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, hw);
+  t.side = make_float2(hw, 0.f);
   if (plateau) {
     const float vljP = fmaf(rc, pc.uP4, -(ac * pc.uP3));
     const bool loP = pc.P2 < r2;
     const float sP = loP ? 1.f : iv;
     const float EP = fmaf(sP, vljP, loP ? w : 0.f);
-    t.rec.w = w - EP;
-    t.side.x = -EP;
+    t.side.x = hw - EP;
     if (!exact) t.side.y = 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
   }
   return t;
@@ -418,7 +423,7 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
 // carries them).  Returns E and h such that dE/dx'_lig = -2 h (x'_lig - x_rec):
 // h = -s d vlj / d dsq (+ gh / sqrt(dsq)); the factor -2 is applied by the caller once per
 // list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
-// pairs beyond the plateau).
+// pairs beyond the plateau).  Without the plateau the side record is not read (k = hw).
 template <bool PLATEAU, bool EXACT>
 __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& h) {
   float u, rs = 0.f;
@@ -431,14 +436,12 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   const float u2 = u * u;
   const float rep = pa.x * u;
   const float t1 = rep - pa.y;
-  const float vlj = t1 * (u2 * u);
+  const float vh = fmaf(t1, u2 * u, pa.w);  // vlj + hw
   // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
   const float dh = fmaf(3.f, t1, rep) * (u2 * u2);
-  const float r2 = fabsf(pa.z);
-  const float iv = copysignf(1.f, pa.z);  // ivor rides on the sign of the rmin2 field
-  const bool lo = dsq < r2;
-  const float s = lo ? 1.f : iv;
-  E = fmaf(s, vlj, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
+  const bool lo = dsq < pa.z;  // always for ivor = +1 (r2 = FLT_MAX)
+  const float s = lo ? 1.f : -1.f;
+  E = fmaf(s, vh, PLATEAU ? ps.x : pa.w);
   h = s * dh;
   if (PLATEAU && !EXACT) h = fmaf(ps.y, rs, h);
 }
@@ -463,12 +466,12 @@ __device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq
 __device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, float d[3]) {
   const float u = 1.f / dsq;
   const float u3 = (u * u) * u;
-  const float iv = pa.z < 0.f ? -1.f : 1.f;
-  const bool lo = dsq < fabsf(pa.z);
-  const float s = lo ? 1.f : iv;
+  const bool neg = pa.z != FLT_MAX;  // ivor = -1
+  const bool lo = dsq < pa.z;
+  const float s = lo ? 1.f : -1.f;
   d[0] = s * (u3 * u);
   d[1] = -s * u3;
-  d[2] = lo ? (iv - 1.f) : 0.f;
+  d[2] = lo && neg ? -2.f : 0.f;  // ivor - 1
 }
 
 // LDS pair-table rows (one per receptor type) have an odd stride of nlt | 1 records, so
@@ -608,7 +611,11 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         dd[cc][2] = bx.z - rb[cc].z;
         dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
         ok[cc] = dsq[cc] < P2;
-        const unsigned pi = ok[cc] ? __umul24((unsigned)__float_as_int(rb[cc].w), stride) + blt : nslots;
+        // the receptor type is read whatever ok is: an empty asm keeps the compiler from
+        // turning the select below into a branch around that read
+        unsigned rt = (unsigned)__float_as_int(rb[cc].w);
+        asm("" : "+v"(rt));
+        const unsigned pi = ok[cc] ? __umul24(rt, stride) + blt : nslots;
         pa[cc] = s_par[pi];
         ps[cc] = PLATEAU ? s_side[pi] : make_float2(0.f, 0.f);
       }
