@@ -573,7 +573,8 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
     // inactive lanes read the all-padding segment at the head of the list array
     const int off = act ? __float_as_int(bl.w) + item * S : 0;
     static_assert(S == 4 || S == 8, "segments of 4 or 8 list entries");
-    float e = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    // -0 is the identity of float addition, so the first pair needs no add
+    float e = -0.f, gx = -0.f, gy = -0.f, gz = -0.f;
     unsigned words[S / 2];
     if (S == 8) {
       const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);

@@ -21,6 +21,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pose-steps/sec batched vs plain at 1e-4 rel energy; p50/p99 dock-request latency"
 HBM_PEAK_GBS = 8000.0
+# VALU issue limit: each of the 4 SIMDs of a CU issues one wave64 VALU instruction per 4
+# cycles (16 lanes a cycle), i.e. one per CU per cycle: 256 CUs x 2.4 GHz peak engine clock
+VALU_PEAK_PER_S = 256 * 2.4e9
 
 
 def log(*a):
@@ -80,12 +83,17 @@ def scoring_rates(dev, wpb):
                     "reduction (C4); best of 3 launches"}
 
 
-def measure_traffic(args, wpb, opt):
-    """HBM traffic of one minimizer launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
-    timed pose-kernel dispatch of a one-launch child run of this workload, one rocprofv3 pass
-    per counter, each scaled by the value the same counter reads for a 1 GiB device copy in
-    the same run (1 GiB read, 1 GiB written).  Returns None when rocprofv3 is unavailable or a
-    pass fails."""
+def _child_command(args, wpb, opt):
+    """A one-launch run of this workload that times nothing else (the profiled child)."""
+    return [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
+            "--no-plain", "--serve-jobs", "0", "--no-traffic", "--no-scoring", "--copy-calibration", "--waves",
+            str(wpb), "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
+            str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps), "--nshell", str(args.nshell)]
+
+
+def _counter_pass(child, ctr):
+    """One rocprofv3 --pmc pass of `child` for counter `ctr`: {dispatch id: (kernel, value)},
+    or None when rocprofv3 is unavailable, times out or writes no counter file."""
     import glob
     import shutil
     import subprocess
@@ -93,41 +101,61 @@ def measure_traffic(args, wpb, opt):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    child = [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-             "--no-plain", "--serve-jobs", "0", "--no-traffic", "--no-scoring", "--copy-calibration", "--waves", str(wpb),
-             "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
-             str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps), "--nshell", str(args.nshell)]
     env = dict(os.environ, TMPDIR="/tmp")
-    tmp = tempfile.mkdtemp(prefix="atj_traffic_", dir="/tmp")
+    tmp = tempfile.mkdtemp(prefix="atj_pmc_", dir="/tmp")
+    log(f"[bench] counter pass {ctr}")
+    proc = subprocess.Popen([exe, "--pmc", ctr, "--output-format", "csv", "-d", tmp, "-o", "c", "--", *child],
+                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env, cwd="/tmp",
+                            start_new_session=True)
+    try:
+        status = proc.wait(timeout=240)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, 9)  # the profiler and the profiled run
+        proc.wait()
+        log(f"[bench] counter pass {ctr} timed out")
+        shutil.rmtree(tmp, ignore_errors=True)
+        return None
+    # the profiler may exit non-zero after writing complete results; the CSV decides
+    files = glob.glob(os.path.join(tmp, "**", "c_counter_collection.csv"), recursive=True)
+    disp = _counter_per_dispatch(files[0], ctr) if files else None
+    shutil.rmtree(tmp, ignore_errors=True)
+    if not disp:
+        log(f"[bench] counter pass {ctr} failed (status {status})")
+    return disp or None
+
+
+def _pose_dispatch(disp):
+    pose = [disp[k][1] for k in sorted(disp) if "pose_kernel" in disp[k][0]]
+    return pose[-1] if pose else None
+
+
+def measure_traffic(args, wpb, opt):
+    """HBM traffic of one minimizer launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
+    timed pose-kernel dispatch of a one-launch child run of this workload, one rocprofv3 pass
+    per counter, each scaled by the value the same counter reads for a 1 GiB device copy in
+    the same run (1 GiB read, 1 GiB written).  Returns None when rocprofv3 is unavailable or a
+    pass fails."""
+    child = _child_command(args, wpb, opt)
     res = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = os.path.join(tmp, ctr)
-        log(f"[bench] traffic pass {ctr}")
-        proc = subprocess.Popen([exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "c", "--", *child],
-                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env, cwd="/tmp",
-                                start_new_session=True)
-        try:
-            status = proc.wait(timeout=240)
-        except subprocess.TimeoutExpired:
-            os.killpg(proc.pid, 9)  # the profiler and the profiled run
-            proc.wait()
-            log(f"[bench] traffic pass {ctr} timed out")
+        disp = _counter_pass(child, ctr)
+        if disp is None:
             return None
-        # the profiler may exit non-zero after writing complete results; the CSV decides
-        files = glob.glob(os.path.join(d, "**", "c_counter_collection.csv"), recursive=True)
-        if not files:
-            log(f"[bench] traffic pass {ctr} failed (status {status})")
-            return None
-        disp = _counter_per_dispatch(files[0], ctr)
         copy = [v for name, v in disp.values() if "copyBuffer" in name]
-        pose = [disp[k][1] for k in sorted(disp) if "pose_kernel" in disp[k][0]]
-        if not copy or not pose or max(copy) <= 0:
+        pose = _pose_dispatch(disp)
+        if not copy or pose is None or max(copy) <= 0:
             return None
         scale = (2 ** 30 / 1024) / max(copy)  # KiB a 1 GiB copy should read (write), over what it reads
-        res[ctr] = (pose[-1] * scale * 1024, scale)
-    shutil.rmtree(tmp, ignore_errors=True)
+        res[ctr] = (pose * scale * 1024, scale)
     return {"fetch_bytes": res["FETCH_SIZE"][0], "write_bytes": res["WRITE_SIZE"][0],
             "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1]}
+
+
+def measure_valu(args, wpb, opt):
+    """VALU wave-instructions of one minimizer launch: SQ_INSTS_VALU of the pose-kernel dispatch
+    of a one-launch child run of this workload (one rocprofv3 pass), or None."""
+    disp = _counter_pass(_child_command(args, wpb, opt), "SQ_INSTS_VALU")
+    return None if disp is None else _pose_dispatch(disp)
 
 
 def main():
@@ -153,7 +181,8 @@ def main():
     ap.add_argument("--no-scoring", action="store_true",
                     help="skip the single-shot energrad (C2) and pair-table adjoint (C4) timings")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 counter passes that measure the minimizer's HBM traffic")
+                    help="skip the rocprofv3 counter passes that measure the minimizer's HBM traffic and VALU "
+                         "instructions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (poses shard trivially across "
@@ -264,6 +293,15 @@ def main():
                            "one-launch run; KiB counts scaled by a 1 GiB device copy measured in the same run")
             log(f"[bench] traffic: {result['roofline']['traffic'] / 1e9:.1f} GB per launch "
                 f"(algorithmic {algo_bytes / 1e9:.1f} GB)")
+        nv = measure_valu(args, system.launch.waves_per_block, system.launch.options)
+        if nv is not None:
+            rate = nv / (k_ms * 1e-3)
+            result["valu_issue"] = {
+                "bound": "valu", "achieved": rate, "peak": VALU_PEAK_PER_S, "unit": "wave64 VALU instructions/s",
+                "frac": rate / VALU_PEAK_PER_S, "instructions_per_launch": nv, "per_pose_step": nv / pose_steps,
+                "method": "rocprofv3 --pmc SQ_INSTS_VALU of the pose-kernel dispatch of a one-launch run, over the "
+                          "timed kernel_ms; peak = 256 CUs x one wave64 VALU issue per cycle x 2.4 GHz"}
+            log(f"[bench] VALU: {nv / pose_steps:.0f} per pose-step, {rate / VALU_PEAK_PER_S:.2f} of the issue limit")
 
     if not args.no_plain:
         nl = min(args.plain_ligands, len(ligs))
