@@ -540,6 +540,13 @@ struct KArgs {
 // lies beyond the plateau and uses the all-zero pair record.
 __device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
 
+// Receptor bead with its type (pair-table row) limited to the table: a type beyond it is
+// read as the last row, so no input can address memory outside the table.
+__device__ __forceinline__ float4 clamp_rec_type(float4 b, int nrt) {
+  b.w = __int_as_float((int)min((unsigned)__float_as_int(b.w), (unsigned)(nrt - 1)));
+  return b;
+}
+
 // Pair loop over the flattened list segments (S entries each) of one chunk of beads.
 // Bead lane j brings its segment count `segs` and first item `excl`; each window of 64
 // items finds its beads by marking the segment starts that fall in the window and taking
@@ -597,7 +604,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         if (REC_LDS) {
           rb[cc] = recp[min(r, nrec)];
         } else {
-          const float4 v = recp[min(r, nrec - 1u)];
+          const float4 v = clamp_rec_type(recp[min(r, nrec - 1u)], a.nrt);
           rb[cc] = r < nrec ? v : sentinel_bead();
         }
       }
@@ -726,9 +733,11 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       xl[0] = b.x;
       xl[1] = b.y;
       xl[2] = b.z;
-      const int tb = __float_as_int(b.w);
-      const int gt = tb & 0xffff;
-      lt = tb >> 16;
+      // types beyond the potential grids or the pair table are read as the last one, so no
+      // input can address memory outside them (the Python layer rejects such ligands)
+      const unsigned tb = (unsigned)__float_as_int(b.w);
+      const int gt = min((int)(tb & 0xffffu), G.ntypes - 1);
+      lt = min((int)(tb >> 16), a.nlt - 1);
       const float* mp = h ? B.M : A.M;
       float M[16];
 #pragma unroll
@@ -966,7 +975,7 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
                           float* s_pgrad) {
   const int tid = threadIdx.x;
   if (s_rec) {
-    for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = a.rec[i];
+    for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = clamp_rec_type(a.rec[i], a.nrt);
     if (tid == 0) s_rec[a.nrec] = sentinel_bead();
   }
   const int ntp = a.nrt * a.nlt;
@@ -1266,7 +1275,7 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
     tb = __float_as_int(b.w);
     transform(M, b.x, b.y, b.z, xp);
   }
-  const int gt = tb & 0xffff, lt = tb >> 16;
+  const int gt = min((int)((unsigned)tb & 0xffffu), a.G.ntypes - 1), lt = min((int)((unsigned)tb >> 16), a.nlt - 1);
   float E, g[3], raw[4];
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0, do_nb = (a.flags & ATJ_TERM_NB) != 0;
   int voxel, cnt, start;
@@ -1284,7 +1293,8 @@ __global__ void bead_terms_kernel(KArgs a, const float4* lig, int nb, const floa
   if (voxel >= 0 && do_nb) {
     for (int c = 0; c < cnt; ++c) {
       const unsigned r = a.G.nblist[start + c];
-      const float4 rb = a.rec[r];
+      if (r >= (unsigned)a.nrec) continue;  // not a bead of this receptor
+      const float4 rb = clamp_rec_type(a.rec[r], a.nrt);
       const float ddx = xp[0] - rb.x, ddy = xp[1] - rb.y, ddz = xp[2] - rb.z;
       const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
       if (dsq < a.G.p2) {
@@ -1360,13 +1370,14 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int n, 
 // plateau gradient projected on the plateau sphere (the complement of the
 // force-shift of test-grid-energrad.py:140-146).
 __global__ void build_potentials_kernel(GridView G, const float4* rec, int nrec, const float* rc, const float* ac,
-                                        const float* ivor, const float* emin, const float* rmin2, int nlt,
-                                        const int* lig_fftype, float cutoff2, float* out, int dx, int dy, int dz,
-                                        int outer) {
+                                        const float* ivor, const float* emin, const float* rmin2, int nrt,
+                                        int nlt, const int* lig_fftype, float cutoff2, float* out, int dx, int dy,
+                                        int dz, int outer) {
   const int t = blockIdx.y;
   const long long nv = (long long)dx * dy * dz;
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lt = lig_fftype[t];
+  // types beyond the pair table are read as its last row / column (memory-safe for any input)
+  const int lt = (int)min((unsigned)lig_fftype[t], (unsigned)(nlt - 1));
   const float P2 = G.p2;
   const float P = sqrtf(P2);
   if (v >= nv) return;
@@ -1387,7 +1398,7 @@ __global__ void build_potentials_kernel(GridView G, const float4* rec, int nrec,
     const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
     const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
     if (!(dsq < cutoff2)) continue;
-    const int pi = __float_as_int(rb.w) * nlt + lt;
+    const int pi = __float_as_int(clamp_rec_type(rb, nrt).w) * nlt + lt;
     const float c_rc = rc[pi], c_ac = ac[pi], iv = ivor[pi], em = emin[pi], r2 = rmin2[pi];
     const float dd = dsq < P2 ? P2 : dsq;
     const float u = 1.f / dd;
@@ -1527,7 +1538,9 @@ int make_view(const AtjGrid* g, GridView& G, bool need_lists) {
   G.dx2 = g->dim2[0];
   G.dy2 = g->dim2[1];
   G.dz2 = g->dim2[2];
-  G.ntypes = g->ntypes;
+  if (g->inner && (g->ntypes <= 0 || g->ntypes > 65536))
+    return fail(ATJ_ERR_INVALID, "ntypes must be 1..65536 with potential grids");
+  G.ntypes = g->ntypes > 0 ? g->ntypes : 1;  // bead grid types are limited to [0, ntypes - 1]
   G.layout = g->layout;
   if (g->nshell < 0 || g->nshell > kMaxShell) return fail(ATJ_ERR_INVALID, "nshell must be 0..6");
   if (need_lists && g->nshell > 0 && !g->nbshell) return fail(ATJ_ERR_INVALID, "nbshell is NULL");
@@ -1825,18 +1838,21 @@ int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff, cons
   if (rc) return rc;
   if (!ff || !rec || !lig_fftype_dev || !inner_out) return fail(ATJ_ERR_INVALID, "NULL argument");
   if (geom->ntypes <= 0 || geom->ntypes > 65535) return fail(ATJ_ERR_INVALID, "bad ntypes");
+  if (ff->nrt <= 0 || ff->nlt <= 0 || !ff->rc || !ff->ac || !ff->ivor || !ff->emin || !ff->rmin2)
+    return fail(ATJ_ERR_INVALID, "empty pair table or NULL pair-table array");
+  if (rec->nrec < 0 || (rec->nrec > 0 && !rec->beads)) return fail(ATJ_ERR_INVALID, "receptor beads NULL");
   const float c2 = cutoff * cutoff;
   const float4* rb = reinterpret_cast<const float4*>(rec->beads);
   long long nv = (long long)G.dx * G.dy * G.dz;
   hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
-                     (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2, ff->nlt,
-                     lig_fftype_dev, c2, inner_out, G.dx, G.dy, G.dz, 0);
+                     (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2, ff->nrt,
+                     ff->nlt, lig_fftype_dev, c2, inner_out, G.dx, G.dy, G.dz, 0);
   HIPCHK(hipGetLastError());
   if (outer_out) {
     nv = (long long)G.dx2 * G.dy2 * G.dz2;
     hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
                        (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2,
-                       ff->nlt, lig_fftype_dev, c2, outer_out, G.dx2, G.dy2, G.dz2, 1);
+                       ff->nrt, ff->nlt, lig_fftype_dev, c2, outer_out, G.dx2, G.dy2, G.dz2, 1);
     HIPCHK(hipGetLastError());
   }
   return ATJ_OK;

@@ -7,8 +7,8 @@ kernels of libattract_amd.so.  Layouts in HBM:
   128-byte record (one cache line per bead per trilinear gather, 8x the plain
   footprint) or ATJ_LAYOUT_PLAIN keeps the reference (type, x, y, z, 4) layout;
 * neighbour lists: per inner voxel (count, start) int32 pairs; the lists are
-  uint16 receptor indices, each list starting on a 16-byte boundary and padded
-  with 0xFFFF to a multiple of 8 entries (one 16-byte load per 8 neighbours);
+  uint16 receptor indices, each non-empty list starting on a 64-byte boundary and
+  padded with 0xFFFF to a multiple of 32 entries (16-byte loads of 8 neighbours);
 * receptor beads: float4 (x, y, z, pair-table row as int32 bits);
 * ligand beads: float4 (x, y, z, grid type | pair-table column << 16 as int32 bits).
 """
@@ -52,12 +52,21 @@ def pack_beads(xyz, tag):
 class Receptor:
     beads: torch.Tensor  # (nrec, 4) float32
     nrec: int
+    max_type: int = None  # largest pair-table row, when known on the host
 
     @classmethod
     def create(cls, coor_rec, rec_atomtypes, device=None):
         """coor_rec (nrec, 3); rec_atomtypes = pair-table row of every bead (compacted types)."""
-        b = pack_beads(coor_rec, rec_atomtypes)
-        return cls(torch.from_numpy(b).to(_dev(device)), len(b))
+        t = np.asarray(rec_atomtypes, np.int64).reshape(-1)
+        if t.min(initial=0) < 0 or t.max(initial=0) > 0x7FFFFFFF:
+            raise ValueError("receptor types out of range")
+        b = pack_beads(coor_rec, t)
+        return cls(torch.from_numpy(b).to(_dev(device)), len(b), int(t.max(initial=0)))
+
+    def check_types(self, ff):
+        """ValueError unless every bead's type is a row of the pair table ff."""
+        if self.max_type is not None and self.max_type >= ff.nrt:
+            raise ValueError(f"receptor type {self.max_type} beyond the {ff.nrt} rows of the pair table")
 
     def struct(self):
         return _lib.AtjReceptor(self.beads.data_ptr() if self.nrec else 0, self.nrec)
@@ -68,6 +77,8 @@ class Ligand:
     beads: torch.Tensor  # (nbeads, 4) float32
     nbeads: int
     coor: np.ndarray = field(repr=False, default=None)
+    max_grid_type: int = None  # largest potential-grid index and pair-table column, when known
+    max_ff_type: int = None
 
     @classmethod
     def create(cls, coor_lig, lig_atomtype_pos, lig_atomtypes, device=None):
@@ -78,7 +89,16 @@ class Ligand:
         if gt.min(initial=0) < 0 or gt.max(initial=0) > 0xFFFF or lt.min(initial=0) < 0 or lt.max(initial=0) > 0x7FFF:
             raise ValueError("ligand types out of range")
         b = pack_beads(coor_lig, gt | (lt << 16))
-        return cls(torch.from_numpy(b).to(_dev(device)), len(b), np.asarray(coor_lig, np.float32).reshape(-1, 3))
+        return cls(torch.from_numpy(b).to(_dev(device)), len(b), np.asarray(coor_lig, np.float32).reshape(-1, 3),
+                   int(gt.max(initial=0)), int(lt.max(initial=0)))
+
+    def check_types(self, grid, ff):
+        """ValueError unless every bead's grid type is a potential grid of `grid` and its
+        force-field type a column of the pair table ff."""
+        if self.max_grid_type is not None and grid.has_potentials and self.max_grid_type >= grid.ntypes:
+            raise ValueError(f"ligand grid type {self.max_grid_type} beyond the {grid.ntypes} potential grids")
+        if self.max_ff_type is not None and self.max_ff_type >= ff.nlt:
+            raise ValueError(f"ligand type {self.max_ff_type} beyond the {ff.nlt} columns of the pair table")
 
     def struct(self):
         return _lib.AtjLigand(self.beads.data_ptr(), self.nbeads, 0)

@@ -69,12 +69,18 @@ def point_terms(coords, grid_types, ff_types, grid, ff=None, receptor=None, flag
     M = c.shape[0]
     gt = np.asarray(grid_types, np.int64).reshape(-1)
     lt = np.asarray(ff_types, np.int64).reshape(-1) if ff_types is not None else np.zeros_like(gt)
-    types = torch.as_tensor((gt | (lt << 16)).astype(np.int32), device=d)
-    if flags is None:
-        flags = _default_flags(dg, terms)
+    if len(gt) and (gt.min() < 0 or lt.min() < 0 or gt.max() > 0xFFFF or lt.max() > 0x7FFF):
+        raise ValueError("point types out of range")
     if ff is None:
         ff = ForceField.create(np.ones((1, 1)), np.ones((1, 1)), np.ones((1, 1)))
     ffd = device_ff(ff)
+    if len(gt) and dg.has_potentials and gt.max() >= dg.ntypes:
+        raise ValueError(f"point grid type {gt.max()} beyond the {dg.ntypes} potential grids")
+    if len(lt) and lt.max() >= ffd.nlt:
+        raise ValueError(f"point type {lt.max()} beyond the {ffd.nlt} columns of the pair table")
+    types = torch.as_tensor((gt | (lt << 16)).astype(np.int32), device=d)
+    if flags is None:
+        flags = _default_flags(dg, terms)
     if receptor is None:
         receptor = Receptor(torch.zeros((1, 4), device=d), 0)
     grid4 = torch.zeros((M, 4), dtype=torch.float32, device=d)
@@ -141,6 +147,7 @@ class ScoringSystem:
         """Re-read the device pointers of grid, pair table and receptor (after any of them
         was replaced); the tensors they point to are kept alive with the structs."""
         g = self.grid
+        self.receptor.check_types(self.ff)
         whole = g.nshell > 0 and not g.shells_valid_for(self.receptor)
         self._gs = g.struct(whole_lists=whole)
         self._fs = self.ff.struct()
@@ -158,6 +165,7 @@ class ScoringSystem:
         begin = 0
         pick = lambda v, i: float(v[i]) if np.ndim(v) else float(v)
         for i, lig in enumerate(ligands):
+            lig.check_types(self.grid, self.ff)
             n = int(pose_counts[i]) if pose_counts is not None else 0
             ns = int(nsteps[i]) if np.ndim(nsteps) else int(nsteps)
             opt = _lib.AtjOptimizer(int(optimizer), pick(lr_rot, i), pick(lr_trans, i), float(beta1), float(beta2),
@@ -194,6 +202,7 @@ class ScoringSystem:
         P = _f32(poses, d)
         N = P.shape[0]
         P = P.reshape(N, 7 if qt else 16)
+        ligand.check_types(self.grid, self.ff)
         nb = ligand.nbeads
         grid4 = torch.zeros((N, nb, 4), dtype=torch.float32, device=d)
         voxel = torch.empty((N, nb), dtype=torch.int32, device=d)

@@ -85,7 +85,8 @@ typedef struct AtjForceField {
   int32_t nlt;
 } AtjForceField;
 
-/* Receptor beads: float4 [nrec] = (x, y, z, ff type as int32 bit pattern). */
+/* Receptor beads: float4 [nrec] = (x, y, z, ff type as int32 bit pattern).  A type of
+ * nrt or more is read as row nrt - 1 of the pair table. */
 typedef struct AtjReceptor {
   const float* beads;
   int32_t nrec;
@@ -93,7 +94,8 @@ typedef struct AtjReceptor {
 
 /* One rigid ligand: float4 [nbeads] = (x, y, z, int32 bit pattern of
  * grid_type | ff_type << 16).  grid_type indexes the potential grids
- * (lig_atomtype_pos), ff_type the ligand column of the pair table. */
+ * (lig_atomtype_pos), ff_type the ligand column of the pair table; types beyond
+ * them are read as the last grid / column. */
 typedef struct AtjLigand {
   const float* beads;
   int32_t nbeads;

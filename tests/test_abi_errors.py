@@ -89,6 +89,17 @@ def test_shell_count_range_and_entries(lib):
     assert _energrad(lib, _grid(nshell=4, nbshell=0)) == INVALID and "nbshell" in _msg(lib)
 
 
+def test_potential_grid_count(lib):
+    assert _energrad(lib, _grid(ntypes=0)) == INVALID and "ntypes" in _msg(lib)
+    assert _energrad(lib, _grid(ntypes=70000)) == INVALID and "ntypes" in _msg(lib)
+
+
+def test_grid_builder_needs_a_pair_table(lib):
+    st = lib.atj_grid_build_potentials(ctypes.byref(_grid()), ctypes.byref(_ff(0, 2)), ctypes.byref(_rec()),
+                                       _addr(dtype=np.int32), 50.0, _addr(), None, None)
+    assert st == INVALID and "pair table" in _msg(lib)
+
+
 def test_grid_term_needs_potentials(lib):
     assert _energrad(lib, _grid(inner=0), flags=_lib.ATJ_TERM_GRID) == INVALID and "potentials" in _msg(lib)
 

@@ -292,6 +292,44 @@ def test_edge_cases(s1):
             s1.system.launch = old
 
 
+def _retyped(beads, field_bits):
+    b = beads.clone()
+    b[:, 3] = torch.from_numpy(np.asarray(field_bits, np.int32).view(np.float32)).to(b.device)
+    return b
+
+
+def test_types_beyond_the_tables_read_as_the_last_one(s1):
+    """Ligand grid / force-field types and receptor types beyond the potential grids and the
+    pair table (which the Python layer rejects, but a C caller can pass) are read as the last
+    grid / row / column, through the LDS and the L1 receptor paths and the point kernel."""
+    from attract_jax_amd.device import Ligand, Receptor
+    from attract_jax_amd.score import ScoringSystem
+    system, lig = s1.system, s1.ligs[0]
+    nt, nrt, nlt = system.grid.ntypes, system.ff.nrt, system.ff.nlt
+    n = lig.nbeads
+    over = Ligand(_retyped(lig.beads, [(nt + 5 + j) | ((nlt + 3 + j) << 16) for j in range(n)]), n)
+    last = Ligand(_retyped(lig.beads, [(nt - 1) | ((nlt - 1) << 16)] * n), n)
+    rec = system.receptor
+    rover = Receptor(_retyped(rec.beads, np.arange(rec.nrec) % 3 + nrt), rec.nrec)
+    rlast = Receptor(_retyped(rec.beads, np.full(rec.nrec, nrt - 1)), rec.nrec)
+    mat = s1.mats(slice(0, 256))
+    for opts in (0, _lib.ATJ_LAUNCH_REC_GLOBAL):
+        launch = _lib.AtjLaunch(8, 0, opts)
+        so = ScoringSystem(system.grid, system.ff, rover, launch=launch)
+        sl = ScoringSystem(system.grid, system.ff, rlast, launch=launch)
+        E1, G1 = so.energrad(mat, over)
+        E2, G2 = sl.energrad(mat, last)
+        assert torch.equal(E1, E2) and torch.equal(G1, G2)
+        assert torch.count_nonzero(E2) > 0
+    a = so.bead_terms(mat[:16], over)
+    b = sl.bead_terms(mat[:16], last)
+    assert all(torch.equal(x, y) for x, y in zip(a, b))
+    with pytest.raises(ValueError):
+        system.energrad(mat, Ligand.create(lig.coor, [nt] * n, [0] * n, DEV))
+    with pytest.raises(ValueError):
+        ScoringSystem(system.grid, system.ff, Receptor.create(s1.w.coor_rec, np.full(rec.nrec, nrt), DEV))
+
+
 def test_receptor_paths_bit_identical(s1):
     """Receptor beads from LDS or through the L1 path: same arithmetic, same results."""
     mat = s1.mats()
