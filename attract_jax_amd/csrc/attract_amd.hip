@@ -532,6 +532,7 @@ struct KArgs {
   float* elast;
   int nsteps_launch;
   unsigned flags;
+  int batch;  // poses per work-counter claim, 1..kBatch
   int* counter;
   unsigned long long* stats;
 };
@@ -1010,10 +1011,10 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
   const int sb = lane >> 3, pi = lane & 7;
   for (;;) {
     int k0 = 0;
-    if (lane == 0) k0 = atomicAdd(a.counter, kBatch);
+    if (lane == 0) k0 = atomicAdd(a.counter, a.batch);
     k0 = __builtin_amdgcn_readfirstlane(k0);
     if (k0 >= a.N) break;
-    const int nbatch = min(kBatch, a.N - k0);
+    const int nbatch = min(a.batch, a.N - k0);
     const bool own = lane < nbatch;
     int p = 0, jid = 0, cnt = 0, nst = 0;
     if (own) {
@@ -1156,16 +1157,16 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
     minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, ws, bs, st, lane);
   } else {
-    // kBatch poses of the processing order per work-counter claim; two consecutive ones of at
+    // a.batch poses of the processing order per work-counter claim; two consecutive ones of at
     // most kHalf beads share the bead phase
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
     float* Ms = s_mat + wave * 24;
     for (;;) {
       int k0 = 0;
-      if (lane == 0) k0 = atomicAdd(a.counter, kBatch);
+      if (lane == 0) k0 = atomicAdd(a.counter, a.batch);
       k0 = __builtin_amdgcn_readfirstlane(k0);
       if (k0 >= a.N) break;
-      const int nbatch = min(kBatch, a.N - k0);
+      const int nbatch = min(a.batch, a.N - k0);
       for (int b = 0; b < nbatch; ++b) {
         int p[2];
         PoseIn P[2];
@@ -1622,19 +1623,41 @@ int device_info(DevInfo& d) {
   return ATJ_OK;
 }
 
+// Poses per work-counter claim for N poses on `waves` wavefronts: the size in 1, 2, 4, 8 with
+// the fewest pose evaluations per wave on the longest path (rounds of claims x size), the
+// largest on a tie (more poses of at most 32 beads then share a bead phase).  Only launches of
+// fewer than 8 poses per wave take fewer than 8 poses per claim.  Each pose's result is the
+// same whatever the claim size.
+int claim_size(long long N, long long waves) {
+  int best = kBatch;
+  long long best_cost = -1;
+  for (int b = kBatch; b >= 1; b >>= 1) {
+    const long long rounds = (N + b * waves - 1) / (b * waves);
+    const long long cost = rounds * b;
+    if (best_cost < 0 || cost < best_cost) {
+      best = b;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
   auto kern = pose_kernel<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>;
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-  if (blocks == 0) {
+  const bool auto_blocks = blocks == 0;
+  if (auto_blocks) {
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
     if (per_cu < 1) per_cu = 1;
     blocks = per_cu * d.cus;
-    // poses a block takes per work-counter fetch round
-    const long long per_block = (long long)wpb * kBatch;
+  }
+  a.batch = claim_size(a.N, (long long)blocks * wpb);
+  if (auto_blocks) {  // no more blocks than the first round of claims needs
+    const long long per_block = (long long)wpb * a.batch;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }

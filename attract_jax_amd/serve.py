@@ -11,6 +11,11 @@ the scheduler
   * runs each receptor group in chunks of `chunk_steps` ADAM steps on its own HIP
     stream, so groups overlap on the device and newly admitted jobs join the next
     chunk of their group (no waiting for the running jobs to finish);
+  * advances, at each chunk boundary, the jobs with the least remaining work first
+    (pose-steps times ligand beads), as many as fill the GPU (`run_poses` poses in
+    flight): a short request overtakes long ones instead of sharing the device with
+    them, and a job that has waited `max_wait` seconds since admission is advanced
+    whatever its size;
   * retires a job when its steps are done: its results are copied to the host and
     its slots and memory are freed.
 The policy (this file's Scheduler and SlotPool) is independent of the device: the
@@ -49,6 +54,7 @@ class Job:
     slot: int = -1
     steps_done: int = 0
     state: str = "pending"     # pending -> admitted -> done | failed
+    t_arrival: float = None    # scheduler clock at submission
     t_admit: float = None
     t_done: float = None
     result: dict = None
@@ -58,6 +64,11 @@ class Job:
     @property
     def remaining(self):
         return self.spec.nsteps - self.steps_done
+
+    @property
+    def remaining_work(self):
+        """Pose-steps left, weighted by ligand beads (the cost of a pose-step grows with them)."""
+        return self.npose * self.remaining * max(1, len(self.spec.ligand_coor))
 
 
 class SlotPool:
@@ -115,10 +126,13 @@ class Group:
 
 
 class Scheduler:
-    def __init__(self, backend, hbm_budget, chunk_steps=25, initial_slots=1 << 16, clock=time.perf_counter):
+    def __init__(self, backend, hbm_budget, chunk_steps=25, initial_slots=1 << 16, clock=time.perf_counter,
+                 run_poses=1 << 16, max_wait=0.5):
         self.backend = backend
         self.budget = int(hbm_budget)
         self.chunk_steps = int(chunk_steps)
+        self.run_poses = int(run_poses)
+        self.max_wait = float(max_wait)
         self.initial_slots = int(initial_slots)
         self.clock = clock
         self.pending = deque()
@@ -130,7 +144,7 @@ class Scheduler:
     # ------------------------------------------------------------------ bookkeeping
     def submit(self, spec, now=None):
         job = Job(len(self.jobs), spec, len(spec.quat))
-        job.spec.arrival = self.clock() if now is None else now
+        job.t_arrival = self.clock() if now is None else now
         self.jobs.append(job)
         self.pending.append(job)
         return job
@@ -191,17 +205,34 @@ class Scheduler:
             g.dirty = True
             self.groups.move_to_end(g.key)
 
+    def _run_set(self, now):
+        """Ids of the jobs to advance: jobs waiting longer than max_wait since admission,
+        then the rest in order of remaining work, until their poses reach run_poses."""
+        runnable = [j for g in self.groups.values() for j in g.jobs if j.remaining > 0]
+        runnable.sort(key=lambda j: (now - j.t_admit <= self.max_wait, j.remaining_work, j.id))
+        chosen, poses = set(), 0
+        for j in runnable:
+            if poses >= self.run_poses:
+                break
+            chosen.add(j.id)
+            poses += j.npose
+        return chosen
+
     def _launch(self, now):
+        chosen = None
         for g in list(self.groups.values()):
             if g.inflight is not None or not g.jobs:
                 continue
-            runnable = [j for j in g.jobs if j.remaining > 0]
-            if not runnable:
+            if chosen is None:
+                chosen = self._run_set(now)
+            jobs = [j for j in g.jobs if j.id in chosen]
+            if not jobs:
                 continue
-            steps = min(self.chunk_steps, max(j.remaining for j in runnable))
-            g.chunk_jobs = list(g.jobs)
+            steps = min(self.chunk_steps, max(j.remaining for j in jobs))
+            dirty = g.dirty or [j.id for j in jobs] != [j.id for j in g.chunk_jobs]
+            g.chunk_jobs = jobs
             g.chunk_steps = steps
-            g.inflight = self.backend.launch(g, g.jobs, steps, g.dirty)
+            g.inflight = self.backend.launch(g, jobs, steps, dirty)
             g.dirty = False
             g.last_used = now
             self.groups.move_to_end(g.key)
@@ -236,7 +267,7 @@ class Scheduler:
         return not self.pending and all(not g.jobs and g.inflight is None for g in self.groups.values())
 
     def latencies(self):
-        return np.array([j.t_done - j.spec.arrival for j in self.done if j.state == "done"])
+        return np.array([j.t_done - j.t_arrival for j in self.done if j.state == "done"])
 
 
 # ---------------------------------------------------------------------------- GPU backend

@@ -144,3 +144,40 @@ def test_slot_pool_growth_between_chunks():
     drive(s, clock)
     assert a.state == b.state == "done"
     assert s.groups[0].pool.capacity >= 50
+
+
+def test_shortest_remaining_work_first():
+    """With room for 100 poses in flight, a short request that arrives while a long one runs
+    is advanced alone until it is done; the long one then resumes."""
+    clock = FakeClock()
+    be = FakeBackend(clock, {0: 1000})
+    s = Scheduler(be, hbm_budget=10 ** 9, chunk_steps=5, initial_slots=2000, clock=clock, run_poses=100)
+    a = s.submit(spec(0, 1000, 200))
+    for _ in range(30):
+        s.tick()
+        clock.t += 1e-3
+    b = s.submit(spec(0, 100, 10))
+    drive(s, clock)
+    assert a.state == b.state == "done" and b.t_done < a.t_done
+    alone = [i for i, (_, ids, _) in enumerate(be.launches) if ids == [b.id]]
+    assert len(alone) == 2  # 10 steps in chunks of 5, without a
+    assert all(ids == [a.id] for _, ids, _ in be.launches[alone[-1] + 1:])
+    assert a.steps_done == 200
+
+
+def test_long_job_is_not_starved():
+    """Short jobs arriving back to back do not hold a long one back beyond max_wait."""
+    clock = FakeClock()
+    be = FakeBackend(clock, {0: 1000})
+    s = Scheduler(be, hbm_budget=10 ** 9, chunk_steps=5, initial_slots=4000, clock=clock, run_poses=100,
+                  max_wait=0.05)
+    a = s.submit(spec(0, 1000, 50))
+    shorts = []
+    for k in range(400):
+        if k % 2 == 0:
+            shorts.append(s.submit(spec(0, 100, 10)))
+        s.tick()
+        clock.t += 1e-3
+    assert a.steps_done > 0  # advanced while short jobs kept coming
+    drive(s, clock)
+    assert a.state == "done" and all(j.state == "done" for j in shorts)
