@@ -404,7 +404,7 @@ class GpuBackend:
         job.handle = None
 
 
-def run_stream(scheduler, specs, poll=0.0005, timeout=600.0):
+def run_stream(scheduler, specs, poll=0.0001, timeout=600.0):
     """Submit specs at their arrival times (seconds from now) and run until all are done.
     Returns (latencies (s), makespan (s), pose-steps)."""
     t0 = scheduler.clock()
