@@ -170,9 +170,6 @@ __device__ __forceinline__ float wave_sum_all(const float acc[kAcc], int lane) {
 __device__ __forceinline__ int acc_index(int l) {
   return ((l >> 5) & 1) | (((l >> 4) & 1) << 1) | (((l >> 3) & 1) << 2) | (((l >> 2) & 1) << 3);
 }
-__host__ __device__ constexpr int acc_lane(int i) {
-  return 32 * (i & 1) + 16 * ((i >> 1) & 1) + 8 * ((i >> 2) & 1) + 4 * (i >> 3);
-}
 
 // Rotation matrix of a (not necessarily normalised) quaternion (x, y, z, w):
 // the polynomial of scipy / jax.scipy Rotation.as_matrix, used by test4.py:399.
@@ -462,9 +459,8 @@ __device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq
   gs = -2.f * h;
 }
 
-// Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs.
-__device__ __forceinline__ void pair_param_grad(float dsq, const float4 pa, float d[3]) {
-  const float u = 1.f / dsq;
+// Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs; u = 1/dsq.
+__device__ __forceinline__ void pair_param_grad(float dsq, float u, const float4 pa, float d[3]) {
   const float u3 = (u * u) * u;
   const bool neg = pa.z != FLT_MAX;  // ivor = -1
   const bool lo = dsq < pa.z;
@@ -559,9 +555,8 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
                                            float wgt, float acc[kAcc], unsigned& npair, int lane) {
   const GridView& G = a.G;
   const float P2 = G.p2;
+  const float uP2 = 1.f / P2;  // (adjoint) the plateau terms' 1/dsq
   const unsigned nrec = (unsigned)a.nrec;
-  const unsigned nlt = (unsigned)a.nlt;
-  const unsigned ntp = (unsigned)(a.nrt * a.nlt);
   const unsigned stride = (unsigned)pair_stride(a.nlt);
   const unsigned nslots = (unsigned)pair_slots(a.nrt, a.nlt);
   int carry = 0;
@@ -613,6 +608,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       float4 pa[4];
       float2 ps[4];
       bool ok[4];
+      unsigned pidx[4];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         dd[cc][0] = bx.x - rb[cc].x;
@@ -625,6 +621,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         unsigned rt = (unsigned)__float_as_int(rb[cc].w);
         asm("" : "+v"(rt));
         const unsigned pi = ok[cc] ? __umul24(rt, stride) + blt : nslots;
+        pidx[cc] = pi;
         pa[cc] = s_par[pi];
         ps[cc] = PLATEAU ? s_side[pi] : make_float2(0.f, 0.f);
       }
@@ -639,18 +636,22 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
         if (MODE == kAdjoint && ok[cc]) {
           float dp[3];
-          pair_param_grad(dsq[cc], pa[cc], dp);
+          pair_param_grad(dsq[cc], __builtin_amdgcn_rcpf(dsq[cc]), pa[cc], dp);
           if (PLATEAU) {
             float dq[3];
-            pair_param_grad(P2, pa[cc], dq);
+            pair_param_grad(P2, uP2, pa[cc], dq);
             dp[0] -= dq[0];
             dp[1] -= dq[1];
             dp[2] -= dq[2];
           }
-          const unsigned pg = (unsigned)__float_as_int(rb[cc].w) * nlt + blt;
+          // the gradient rows have the pair table's odd stride, so lanes adding to one ligand
+          // type under different receptor types hit different LDS banks
+          const unsigned pg = pidx[cc];
           atomicAdd(&s_pgrad[pg], wgt * dp[0]);
-          atomicAdd(&s_pgrad[ntp + pg], wgt * dp[1]);
-          atomicAdd(&s_pgrad[2 * ntp + pg], wgt * dp[2]);
+          atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
+          // the emin derivative is zero except for repulsive pairs inside rmin: LDS float
+          // atomics are slow, so the zero ones are not issued
+          if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
         }
       }
     }
@@ -979,7 +980,6 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
     for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = clamp_rec_type(a.rec[i], a.nrt);
     if (tid == 0) s_rec[a.nrec] = sentinel_bead();
   }
-  const int ntp = a.nrt * a.nlt;
   const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
   const PlateauConst pc = plateau_const(a.G.p2);
   for (int i = tid; i <= nslots; i += blockDim.x) {
@@ -995,7 +995,7 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
     s_side[i] = t.side;
   }
   if (s_pgrad)
-    for (int i = tid; i < 3 * ntp; i += blockDim.x) s_pgrad[i] = 0.f;
+    for (int i = tid; i < 3 * nslots; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
 // Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
@@ -1140,7 +1140,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int nslots = pair_slots(a.nrt, a.nlt);
   // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
   // wave scratch [nwaves] | minimizer batches [nwaves], or pose matrices [nwaves][2][12] and
-  // (adjoint) the pair-table gradient [3][ntp]
+  // (adjoint) the pair-table gradient [3][nslots], rows strided as the pair records
   float4* s_par = smem;
   float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
   float4* s_rec = reinterpret_cast<float4*>(s_side + side_slots(nslots));
@@ -1224,7 +1224,11 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   if (MODE == kAdjoint) {
     __syncthreads();
     float* out = a.partial + (size_t)blockIdx.x * 3 * ntp;
-    for (int i = threadIdx.x; i < 3 * ntp; i += blockDim.x) out[i] = s_pgrad[i];
+    const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
+    for (int i = threadIdx.x; i < 3 * ntp; i += blockDim.x) {
+      const int c = i / ntp, k = i - c * ntp, rt = k / a.nlt;
+      out[i] = s_pgrad[c * nslots + rt * stride + (k - rt * a.nlt)];
+    }
   }
 }
 
@@ -1353,12 +1357,20 @@ __global__ void optimizer_step_kernel(const float* dEdM, float* energy, int N, f
   count[p] = cnt;
 }
 
+// Column sums of partial [blocks][n] in a fixed order (deterministic): a block of 256
+// threads takes 64 columns; thread (c, r) sums rows r, r + 4, ... of column c, and the four
+// row groups are added in order.
+constexpr int kReduceCols = 64, kReduceRows = 4;
 __global__ void reduce_partials_kernel(const float* partial, int blocks, int n, float* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  __shared__ float part[kReduceRows][kReduceCols];
+  const int c = threadIdx.x % kReduceCols, r = threadIdx.x / kReduceCols;
+  const int i = blockIdx.x * kReduceCols + c;
   float s = 0.f;
-  for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * n + i];
-  out[i] = s;
+  if (i < n)
+    for (int b = r; b < blocks; b += kReduceRows) s += partial[(size_t)b * n + i];
+  part[r][c] = s;
+  __syncthreads();
+  if (r == 0 && i < n) out[i] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1695,11 +1707,10 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   const uint32_t options = launch ? launch->options : 0u;
   if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
-  const size_t ntp = (size_t)a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
                       wpb * sizeof(WaveScratch) + (MODE == kMinimize ? wpb * sizeof(BatchScratch) : wpb * 24 * 4) +
-                      (MODE == kAdjoint ? 3 * ntp * 4 : 0);
+                      (MODE == kAdjoint ? 3 * (size_t)nslots * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
   const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
@@ -1789,8 +1800,8 @@ int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const Atj
 
 int atj_reduce_partials(const float* partial, int32_t blocks, int32_t n, float* out, void* stream) {
   if (!partial || !out || blocks <= 0 || n <= 0) return fail(ATJ_ERR_INVALID, "bad reduce arguments");
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
-                     blocks, n, out);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((n + kReduceCols - 1) / kReduceCols),
+                     dim3(kReduceCols * kReduceRows), 0, (hipStream_t)stream, partial, blocks, n, out);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }
