@@ -274,6 +274,31 @@ def test_c2_scale_subset(s2):
     assert np.array_equal(vox, s2.twins[0].bead_terms(mat[:512])["voxel"])
 
 
+def test_claim_sizes_give_identical_results(s1, s2):
+    """On 32 wavefronts (8 blocks of 4), launches of 20, 40, 100 and 2048 poses claim 1, 2, 4 and
+    8 poses per atomic; every pose's energy, gradient and minimized pose are the same, bit for
+    bit, whatever the claim size."""
+    small = _lib.AtjLaunch(4, 8, 0)
+    old2, old1 = s2.system.launch, s1.system.launch
+    try:
+        s2.system.launch = small
+        mat = s2.mats()
+        E, G = s2.system.energrad(mat, s2.ligs[0])
+        for n in (20, 40, 100):
+            En, Gn = s2.system.energrad(mat[:n], s2.ligs[0])
+            assert torch.equal(En, E[:n]) and torch.equal(Gn, G[:n]), n
+        s1.system.launch = small
+        w = s1.w
+        full = PoseState.create(w.quat, w.trans, DEV)
+        minimize(s1.system, BatchPlan(s1.system, s1.ligs, [w.npose], 12, w.lr_rot, w.lr_trans), full, 12)
+        for n in (20, 40):
+            part = PoseState.create(w.quat[:n], w.trans[:n], DEV)
+            minimize(s1.system, BatchPlan(s1.system, s1.ligs, [n], 12, w.lr_rot, w.lr_trans), part, 12)
+            assert torch.equal(part.pose, full.pose[:n]) and torch.equal(part.energy, full.energy[:n]), n
+    finally:
+        s2.system.launch, s1.system.launch = old2, old1
+
+
 def test_edge_cases(s1):
     w = s1.w
     E, G = s1.system.energrad(np.zeros((0, 4, 4), np.float32), s1.ligs[0])
