@@ -1139,7 +1139,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int ntp = a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
   // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
-  // wave scratch [nwaves] | minimizer batches [nwaves], or pose matrices [nwaves][2][12] and
+  // wave scratch [nwaves] | minimizer batches [nwaves], or pose matrices [nwaves][kBatch][12] and
   // (adjoint) the pair-table gradient [3][nslots], rows strided as the pair records
   float4* s_par = smem;
   float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
@@ -1147,7 +1147,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_mat = reinterpret_cast<float*>(s_ws + nwaves);
-  float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * 24 : nullptr;
+  float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * (kBatch * 12) : nullptr;
   stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
@@ -1158,51 +1158,53 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, ws, bs, st, lane);
   } else {
     // a.batch poses of the processing order per work-counter claim; two consecutive ones of at
-    // most kHalf beads share the bead phase
+    // most kHalf beads share the bead phase.  Lane b fetches pose b's index, weight and matrix
+    // for the whole claim at once (one round of memory latency per claim, not per pose).
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
-    float* Ms = s_mat + wave * 24;
+    float* Ms = s_mat + wave * (kBatch * 12);
     for (;;) {
       int k0 = 0;
       if (lane == 0) k0 = atomicAdd(a.counter, a.batch);
       k0 = __builtin_amdgcn_readfirstlane(k0);
       if (k0 >= a.N) break;
       const int nbatch = min(a.batch, a.N - k0);
+      int p_l = 0;
+      float w_l = 0.f;
+      wave_sync();  // the previous claim's readers of Ms are done
+      if (lane < nbatch) {
+        p_l = a.pose_order ? a.pose_order[k0 + lane] : k0 + lane;
+        if (MODE == kAdjoint) w_l = a.weights[p_l];
+        float Mp[16];
+        load_pose_matrix(a.poses + (size_t)p_l * (qt ? 7 : 16), qt, Mp);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) Ms[12 * lane + 3 * k + l] = Mp[4 * k + l];
+      }
+      wave_sync();
       for (int b = 0; b < nbatch; ++b) {
         int p[2];
         PoseIn P[2];
-        p[0] = a.pose_order ? a.pose_order[k0 + b] : k0 + b;
+        p[0] = __builtin_amdgcn_readlane(p_l, b);
         const AtjJob job = a.jobs[a.pose_job ? a.pose_job[p[0]] : 0];
         P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
         P[0].nb = job.ligand.nbeads;
+        P[0].M = Ms + 12 * b;
+        P[0].wgt = MODE == kAdjoint ? lane_value(w_l, b) : 0.f;
         P[1] = P[0];
         P[1].nb = 0;
         p[1] = -1;
         if (P[0].nb <= kHalf && b + 1 < nbatch) {
-          const int q = a.pose_order ? a.pose_order[k0 + b + 1] : k0 + b + 1;
+          const int q = __builtin_amdgcn_readlane(p_l, b + 1);
           const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[q] : 0];
           if (job2.ligand.nbeads <= kHalf) {
             p[1] = q;
             P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
             P[1].nb = job2.ligand.nbeads;
+            P[1].M = Ms + 12 * (b + 1);
+            P[1].wgt = MODE == kAdjoint ? lane_value(w_l, b + 1) : 0.f;
           }
         }
-        wave_sync();  // the previous evaluation's readers of Ms are done
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          P[h].M = Ms + 12 * h;
-          P[h].wgt = 0.f;
-          if (h == 1 && p[1] < 0) continue;
-          float Mp[16];
-          load_pose_matrix(a.poses + (size_t)p[h] * (qt ? 7 : 16), qt, Mp);
-          if (MODE == kAdjoint) P[h].wgt = a.weights[p[h]];
-          if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-              for (int l = 0; l < 3; ++l) Ms[12 * h + 3 * k + l] = Mp[4 * k + l];
-          }
-        }
-        wave_sync();
         float u[2];
         pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, P[0], P[1], st,
                                                              lane, u[0], u[1]);
@@ -1709,7 +1711,8 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
-                      wpb * sizeof(WaveScratch) + (MODE == kMinimize ? wpb * sizeof(BatchScratch) : wpb * 24 * 4) +
+                      wpb * sizeof(WaveScratch) +
+                      (MODE == kMinimize ? wpb * sizeof(BatchScratch) : wpb * kBatch * 12 * sizeof(float)) +
                       (MODE == kAdjoint ? 3 * (size_t)nslots * 4 : 0);
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
