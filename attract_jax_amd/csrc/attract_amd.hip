@@ -528,7 +528,8 @@ struct KArgs {
   float* elast;
   int nsteps_launch;
   unsigned flags;
-  int batch;  // poses per work-counter claim, 1..kBatch
+  int batch;          // poses per work-counter claim, 1..kBatch
+  int static_blocks;  // blocks whose wavefronts take their first claim by index (the resident ones)
   int* counter;
   unsigned long long* stats;
 };
@@ -998,6 +999,28 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
     for (int i = tid; i < 3 * nslots; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
+// Work claims of a persistent wavefront, `a.batch` poses each.  In the blocks the device holds
+// at once (a.static_blocks) a wavefront's first claim is fixed by its index; every other claim
+// comes from the work counter and follows those first claims.  A launch whose first claims
+// cover every pose thus takes no atomics (one per wavefront on a single address made small
+// launches atomic-bound), and a larger one takes none for its first round.  Returns the
+// claim's first pose (>= N: no work left).
+__device__ __forceinline__ long long first_round(const KArgs& a) {
+  return (long long)a.static_blocks * (blockDim.x / kWave) * a.batch;
+}
+__device__ __forceinline__ long long next_claim(const KArgs& a, int lane) {
+  const long long fr = first_round(a);
+  if (fr >= a.N) return a.N;
+  int c = 0;
+  if (lane == 0) c = atomicAdd(a.counter, a.batch);
+  return fr + __builtin_amdgcn_readfirstlane(c);
+}
+__device__ __forceinline__ long long first_claim(const KArgs& a, int lane) {
+  if ((int)blockIdx.x >= a.static_blocks) return next_claim(a, lane);
+  const int wave = uni((int)(threadIdx.x / kWave));  // uniform across the wavefront
+  return ((long long)blockIdx.x * (blockDim.x / kWave) + wave) * a.batch;
+}
+
 // Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
 // processing order at a time; per optimizer step it evaluates the slots still stepping
 // in order (all 64 lanes on one pose, or a slot of at most kHalf beads together with the
@@ -1009,11 +1032,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
                               int lane) {
   // Slot lanes of the optimizer update: lane 8 b + i updates parameter i (< 7) of slot b.
   const int sb = lane >> 3, pi = lane & 7;
-  for (;;) {
-    int k0 = 0;
-    if (lane == 0) k0 = atomicAdd(a.counter, a.batch);
-    k0 = __builtin_amdgcn_readfirstlane(k0);
-    if (k0 >= a.N) break;
+  for (long long claim = first_claim(a, lane); claim < a.N; claim = next_claim(a, lane)) {
+    const int k0 = (int)claim;
     const int nbatch = min(a.batch, a.N - k0);
     const bool own = lane < nbatch;
     int p = 0, jid = 0, cnt = 0, nst = 0;
@@ -1162,11 +1182,8 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a
     // for the whole claim at once (one round of memory latency per claim, not per pose).
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
     float* Ms = s_mat + wave * (kBatch * 12);
-    for (;;) {
-      int k0 = 0;
-      if (lane == 0) k0 = atomicAdd(a.counter, a.batch);
-      k0 = __builtin_amdgcn_readfirstlane(k0);
-      if (k0 >= a.N) break;
+    for (long long claim = first_claim(a, lane); claim < a.N; claim = next_claim(a, lane)) {
+      const int k0 = (int)claim;
       const int nbatch = min(a.batch, a.N - k0);
       int p_l = 0;
       float w_l = 0.f;
@@ -1662,19 +1679,19 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
+  if (per_cu < 1) per_cu = 1;
+  const int resident = per_cu * d.cus;
   const bool auto_blocks = blocks == 0;
-  if (auto_blocks) {
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wpb * kWave, lds));
-    if (per_cu < 1) per_cu = 1;
-    blocks = per_cu * d.cus;
-  }
-  a.batch = claim_size(a.N, (long long)blocks * wpb);
+  if (auto_blocks) blocks = resident;
+  a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
   if (auto_blocks) {  // no more blocks than the first round of claims needs
     const long long per_block = (long long)wpb * a.batch;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
+  a.static_blocks = min(blocks, resident);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(wpb * kWave), lds, stream, a);
