@@ -1,7 +1,8 @@
 // libattract_amd: ATTRACT rigid-body docking scoring and minimization for gfx950.
 //
-// One wavefront (64 lanes) evaluates one pose at a time and claims poses from a
-// device work counter, 8 per claim, until the list is exhausted (persistent kernel).  For
+// One wavefront (64 lanes) evaluates one pose at a time and takes claims of up to 8
+// poses -- its first by its index, later ones from a device work counter -- until the
+// list is exhausted (persistent kernel).  For
 // each chunk of up to 64 ligand beads:
 //   * bead phase  -- lane = bead: rigid transform in registers, grid trilinear
 //     term (one 128-byte record per bead in the corner-packed layout), voxel of
