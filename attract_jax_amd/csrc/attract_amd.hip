@@ -1386,8 +1386,11 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int n, 
   const int c = threadIdx.x % kReduceCols, r = threadIdx.x / kReduceCols;
   const int i = blockIdx.x * kReduceCols + c;
   float s = 0.f;
-  if (i < n)
+  if (i < n) {
+    // unrolled so that eight loads are in flight together; the additions keep their order
+#pragma unroll 8
     for (int b = r; b < blocks; b += kReduceRows) s += partial[(size_t)b * n + i];
+  }
   part[r][c] = s;
   __syncthreads();
   if (r == 0 && i < n) out[i] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
