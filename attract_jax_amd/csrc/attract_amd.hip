@@ -54,6 +54,9 @@ constexpr int kHalf = 32;          // ligands of at most this many beads take ha
 constexpr int kMaxShell = 6;       // most list shells per grid point (16-bit counts in one int4)
 constexpr double kShellMargin = 1e-3;  // A added to every shell radius: covers FP32 rounding of distances
 constexpr int kMaxWavesPerBlock = 16;
+// The adjoint's pair loop holds more values (the LDS gradient indices and terms): its blocks
+// have at most 12 wavefronts, 3 per SIMD, so that it may use up to 168 VGPRs without spills.
+constexpr int kMaxAdjointWaves = 12;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
 constexpr int kPairCost = 30;      // approximate instructions per neighbour slot
 constexpr int kItemCost = 90;      // approximate instructions per list segment
@@ -1152,7 +1155,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
 }
 
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
-__global__ void __launch_bounds__(kWave * kMaxWavesPerBlock) pose_kernel(KArgs a) {
+__global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : kMaxWavesPerBlock))
+    pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
   const int nwaves = blockDim.x / kWave;
   const int wave = threadIdx.x / kWave;
@@ -1728,6 +1732,7 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (rc) return rc;
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
+  if (MODE == kAdjoint) wpb = min(wpb, kMaxAdjointWaves);
   const uint32_t options = launch ? launch->options : 0u;
   if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
   const int nslots = pair_slots(a.nrt, a.nlt);
