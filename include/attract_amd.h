@@ -148,10 +148,13 @@ typedef struct AtjPoseState {
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
 #define ATJ_LAUNCH_REC_GLOBAL 0x1u  /* read receptor beads through the vector-memory (L1) path, not LDS */
 typedef struct AtjLaunch {
-  int32_t waves_per_block;  /* 1..16 (0 = default 4) */
-  int32_t blocks;           /* 0 = one persistent wave per pose up to the device fill */
+  int32_t waves_per_block;  /* 1..16 (0 = default 4); the adjoint uses at most 12 */
+  int32_t blocks;           /* 0 = as many as the device holds, fewer for small launches */
   uint32_t options;         /* ATJ_LAUNCH_* */
 } AtjLaunch;
+/* Poses are claimed 1, 2, 4 or 8 at a time (chosen from N and the wavefronts launched); the
+ * first claim of each wavefront in the blocks the device holds at once is fixed by its
+ * index, further claims come from the work counter. */
 
 const char* atj_last_error(void);
 const char* atj_version(void);
