@@ -58,3 +58,44 @@ def test_full_batched_minimization_properties():
         xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
         r = O.rmsd(xg, xt)
         assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.75, (k, r)
+
+
+def test_full_single_shot_scoring_and_adjoint_properties():
+    """The 131,072-pose energrad: a sample matches the twin (voxels bit for bit, energies within
+    1e-4) and scoring that sample alone gives the same bits.  The 65,536-pose pair-table adjoint:
+    a rerun agrees to the rounding of its per-block sums (LDS atomics add in no fixed order), and
+    it is linear in the pose weights (g(w1 + w2) = g(w1) + g(w2) to rounding)."""
+    from attract_jax_amd.autodiff import param_adjoint
+    from attract_jax_amd.workloads import c2, c4
+    w = c2()
+    system, ligs = build_system(w, DEV, keep_plain=True)
+    mat = mat_from_qt(w.quat, w.trans).astype(np.float32)
+    assert len(mat) == 131072
+    E, G = system.energrad(mat, ligs[0])
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(len(mat), 1024, replace=False))
+    Es, Gs = system.energrad(mat[idx], ligs[0])
+    assert torch.equal(Es, E[idx]) and torch.equal(Gs, G[idx])
+    grid = system.grid.to_reference_grid()
+    f = system.ff
+    ffa = [t.cpu().numpy() for t in (f.rc, f.ac, f.ivor, f.emin, f.rmin2)]
+    lig = w.ligands[0]
+    twin = Twin(grid, ffa, w.coor_rec, w.rec_types - 1, lig.coor, lig.types - 1, lig.types - 1)
+    Et, Gt = twin.energrad(mat[idx])
+    En = Es.cpu().numpy()
+    assert (np.abs(En - Et) / np.maximum(np.abs(Et), 1.0)).max() < 1e-4
+    vox = system.bead_terms(mat[idx], ligs[0])[1].cpu().numpy()
+    assert np.array_equal(vox, twin.bead_terms(mat[idx])["voxel"])
+
+    w4 = c4()
+    m4 = torch.from_numpy(mat_from_qt(w4.quat, w4.trans).astype(np.float32)).to(DEV).reshape(-1, 16)
+    assert m4.shape[0] == 65536
+    w1 = torch.from_numpy(rng.uniform(0.5, 1.5, 65536).astype(np.float32)).to(DEV)
+    w2 = torch.from_numpy(rng.uniform(-1.0, 1.0, 65536).astype(np.float32)).to(DEV)
+    g1 = param_adjoint(system, ligs[0], m4, system.ff, w1, system.flags)
+    g1b = param_adjoint(system, ligs[0], m4, system.ff, w1, system.flags)
+    assert g1.abs().max().item() > 0
+    assert ((g1 - g1b).abs().max() / g1.abs().max()).item() < 1e-5
+    g2 = param_adjoint(system, ligs[0], m4, system.ff, w2, system.flags)
+    g12 = param_adjoint(system, ligs[0], m4, system.ff, w1 + w2, system.flags)
+    assert ((g12 - g1 - g2).abs().max() / (g1.abs() + g2.abs()).max()).item() < 1e-5
