@@ -99,3 +99,33 @@ def test_full_single_shot_scoring_and_adjoint_properties():
     g2 = param_adjoint(system, ligs[0], m4, system.ff, w2, system.flags)
     g12 = param_adjoint(system, ligs[0], m4, system.ff, w1 + w2, system.flags)
     assert ((g12 - g1 - g2).abs().max() / (g1.abs() + g2.abs()).max()).item() < 1e-5
+
+
+def test_full_size_grid_build_samples():
+    """The C2 grid (96^3 inner, 64^3 outer, 32 types, 2000 receptor beads) built on the GPU:
+    sampled grid points' potentials match the FP64 oracle's sum over receptor beads, and their
+    neighbour lists hold exactly the beads within neighbourdis (up to FP32 rounding at the edge)."""
+    from attract_jax_amd.workloads import c2
+    w = c2(npose=8)
+    system, _ = build_system(w, DEV, keep_plain=True)
+    g = system.grid.to_reference_grid()
+    f = system.ff
+    ff = O.FF(*[t.cpu().numpy().astype(np.float64) for t in (f.rc, f.ac, f.ivor, f.emin, f.rmin2)])
+    rng = np.random.default_rng(11)
+    idx = np.stack([rng.integers(0, d, 300) for d in g.dim], 1)
+    pts = g.origin + g.gridspacing * idx
+    for t in (0, 13, 31):
+        ref = O.grid_point_potential(pts, w.coor_rec, w.rec_types - 1, t, ff, g.plateaudis)
+        got = g.inner_potential_grid[t][idx[:, 0], idx[:, 1], idx[:, 2]]
+        assert (np.abs(got - ref) / np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)).max() < 1e-4, t
+    counts, starts, lists = O.csr_lists(g)
+    flat = np.ravel_multi_index(idx.T, g.dim)
+    nonempty = 0
+    for v, p in zip(flat, pts):
+        ref = O.grid_point_neighbours(p, w.coor_rec, g.neighbourdis)
+        got = np.sort(lists[starts[v]:starts[v] + counts[v]])
+        nonempty += len(ref) > 0
+        if not np.array_equal(got, ref):
+            dd = np.linalg.norm(w.coor_rec[np.setxor1d(got, ref).astype(int)] - p, axis=1)
+            assert np.all(np.abs(dd - g.neighbourdis) < 1e-4)
+    assert nonempty > 20
