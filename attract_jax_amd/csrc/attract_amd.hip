@@ -382,19 +382,20 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, r2, hw) and 8-byte side record (k, gh) of one pair type.
+// 16-byte pair record (rc, ac, r2, kl) and 8-byte side record (kh, gh) of one pair type.
 // ATTRACT's pair energy (test-grid-energrad.py:95-111) is vlj + (ivor - 1) emin for
 // dsq < rmin2 and ivor vlj beyond, ivor = +-1 (ipon, attract-original/
 // convert-attract-par.py:14-15); an ivor of another magnitude poisons the record (NaN
-// energies) rather than being misread.  With w = (ivor - 1) emin, lo = dsq < rmin2 and
-// s = lo ? 1 : ivor, both branches are s (vlj + w / 2) + w / 2: the record holds hw = w / 2
-// and k = w / 2 (minus E(plateaudis) with the plateau), so that E = s (vlj + hw) + k needs
-// no per-pair choice of constants.  For ivor = +1, w = 0 and s = 1 on both sides of rmin2,
-// so the record stores r2 = FLT_MAX and lo always holds; for ivor = -1 it stores
-// r2 = rmin2 (infinity for rmin2 >= FLT_MAX), and s = lo ? 1 : -1 either way.  The
-// gradient factor is -2 (s dh + gh / sqrt(dsq)) (see pair_eval), gh = P dE/ddsq (P) being
-// the force-shift factor -2 P dE/ddsq (P) (test-grid-energrad.py:124-147) divided by -2;
-// zero for the exact derivative and without the plateau.
+// energies) rather than being misread.  With
+// w = (ivor - 1) emin, lo = dsq < rmin2 and s = lo ? 1 : ivor, both branches are s vlj + k
+// with k = kl = w on the low side and k = kh = 0 beyond (each minus E(plateaudis) with the
+// plateau); vlj = t1 u^3, so E = t1 (s u^3) + k, and s folds into u^3 by one select.  For
+// ivor = +1, w = 0 and s = 1 on both sides of rmin2, so the record stores r2 = FLT_MAX and
+// lo always holds; for ivor = -1 it stores r2 = rmin2 (infinity for rmin2 >= FLT_MAX), and
+// s = lo ? 1 : -1 either way.  The gradient factor is -2 (s dh + gh / sqrt(dsq)) (see
+// pair_eval), gh = P dE/ddsq (P) being the force-shift factor -2 P dE/ddsq (P)
+// (test-grid-energrad.py:124-147) divided by -2; zero for the exact derivative and without
+// the plateau.  Without the plateau kh = 0 and the side record is not read.
 struct PairType {
   float4 rec;
   float2 side;
@@ -404,16 +405,16 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
                                               bool exact, const PlateauConst& pc) {
   if (fabsf(iv) != 1.f) rc = __int_as_float(0x7fc00000);
   const float w = (iv - 1.f) * em;
-  const float hw = 0.5f * w;
   PairType t;
-  t.rec = make_float4(rc, ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, hw);
-  t.side = make_float2(hw, 0.f);
+  t.rec = make_float4(rc, ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, w);
+  t.side = make_float2(0.f, 0.f);
   if (plateau) {
     const float vljP = fmaf(rc, pc.uP4, -(ac * pc.uP3));
     const bool loP = pc.P2 < r2;
     const float sP = loP ? 1.f : iv;
     const float EP = fmaf(sP, vljP, loP ? w : 0.f);
-    t.side.x = hw - EP;
+    t.rec.w = w - EP;
+    t.side.x = -EP;
     if (!exact) t.side.y = 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
   }
   return t;
@@ -424,7 +425,7 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
 // carries them).  Returns E and h such that dE/dx'_lig = -2 h (x'_lig - x_rec):
 // h = -s d vlj / d dsq (+ gh / sqrt(dsq)); the factor -2 is applied by the caller once per
 // list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
-// pairs beyond the plateau).  Without the plateau the side record is not read (k = hw).
+// pairs beyond the plateau).  Without the plateau the side record is not read (kh = 0).
 template <bool PLATEAU, bool EXACT>
 __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& h) {
   float u, rs = 0.f;
@@ -437,13 +438,12 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   const float u2 = u * u;
   const float rep = pa.x * u;
   const float t1 = rep - pa.y;
-  const float vh = fmaf(t1, u2 * u, pa.w);  // vlj + hw
-  // d vlj / d dsq = -(4 rc u - 3 ac) u^4 = -(3 t1 + rep) u^4
-  const float dh = fmaf(3.f, t1, rep) * (u2 * u2);
   const bool lo = dsq < pa.z;  // always for ivor = +1 (r2 = FLT_MAX)
-  const float s = lo ? 1.f : -1.f;
-  E = fmaf(s, vh, PLATEAU ? ps.x : pa.w);
-  h = s * dh;
+  const float u3 = u2 * u;
+  const float su3 = lo ? u3 : -u3;  // s u^3
+  E = fmaf(t1, su3, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
+  // s d vlj / d dsq = -(4 rc u - 3 ac) s u^4 = -(3 t1 + rep) s u^4
+  h = fmaf(3.f, t1, rep) * (su3 * u);
   if (PLATEAU && !EXACT) h = fmaf(ps.y, rs, h);
 }
 
