@@ -1620,7 +1620,7 @@ int fill_common(KArgs& a, const AtjGrid* grid, const AtjForceField* ff, const At
   if ((flags & ATJ_TERM_GRID) && !grid->inner)
     return fail(ATJ_ERR_INVALID, "grid term requested but the grid has no potentials");
   if (ff->nrt <= 0 || ff->nlt <= 0) return fail(ATJ_ERR_INVALID, "empty pair table");
-  if (ff->nrt * ff->nlt > 4096) return fail(ATJ_ERR_UNSUPPORTED, "pair table larger than 4096 entries");
+  if ((long long)ff->nrt * ff->nlt > 4096) return fail(ATJ_ERR_UNSUPPORTED, "pair table larger than 4096 entries");
   if (!ff->rc || !ff->ac || !ff->ivor || !ff->emin || !ff->rmin2) return fail(ATJ_ERR_INVALID, "pair table array is NULL");
   if (rec->nrec < 0 || rec->nrec > 65535) return fail(ATJ_ERR_UNSUPPORTED, "receptor must have < 65536 beads");
   if (rec->nrec > 0 && !rec->beads) return fail(ATJ_ERR_INVALID, "receptor beads NULL");

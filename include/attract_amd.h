@@ -74,7 +74,12 @@ typedef struct AtjGrid {
 } AtjGrid;
 
 /* ATTRACT pair table, compacted to receptor types x ligand types
- * (test-grid-energrad.py:67-79).  All arrays are float32 [nrt][nlt], row-major. */
+ * (test-grid-energrad.py:67-79).  All arrays are float32 [nrt][nlt], row-major.  The pose
+ * kernels stage it into each block's LDS as 24 bytes per slot, nrt x (nlt | 1) slots, next
+ * to per-wavefront scratch (about 4.3 KB per wavefront for the minimizer, 3 KB for energrad;
+ * the adjoint adds 12 bytes per slot).  A table that does not fit in 160 KB beside them
+ * returns ATJ_ERR_UNSUPPORTED: at 16 wavefronts per block the minimizer takes up to about
+ * 3,900 slots (61 receptor x 61 ligand types), fewer wavefronts per block leave room for more. */
 typedef struct AtjForceField {
   const float* rc;
   const float* ac;

@@ -107,6 +107,8 @@ def test_grid_term_needs_potentials(lib):
 def test_pair_table_and_receptor_limits(lib):
     assert _energrad(lib, _grid(), ff=_ff(0, 2)) == INVALID and "pair table" in _msg(lib)
     assert _energrad(lib, _grid(), ff=_ff(65, 64)) == UNSUPPORTED and "4096" in _msg(lib)
+    # 65536 x 65536 is 0 in 32-bit arithmetic: the size check must not wrap
+    assert _energrad(lib, _grid(), ff=_ff(65536, 65536)) == UNSUPPORTED and "4096" in _msg(lib)
     assert _energrad(lib, _grid(), rec=_lib.AtjReceptor(_addr(), 70000)) == UNSUPPORTED and "65536" in _msg(lib)
     assert _energrad(lib, _grid(), rec=_lib.AtjReceptor(None, 10)) == INVALID and "receptor" in _msg(lib)
 
