@@ -19,13 +19,17 @@ PINS = {
     "transform": "identity / pure translation invariants; as_matrix vs scipy.spatial.transform",
     "trilinear": "affine fields reproduced exactly; integer coordinate -> 0; outside -> 0",
     "grid_energrad": "same as trilinear + gradient sign convention (-interpolated stored gradient)",
-    "nb_energrad": "brute-force all-pairs within plateaudis on tiny systems; force-shift closed form",
+    "nb_energrad": "brute-force all-pairs within plateaudis on tiny systems; force-shift closed form; "
+                   "the reference's own outputs test-nb-energrad.out / test-nb-energrad-plateau.out on "
+                   "mono-leu (nb_energrad_complete)",
+    "nb_energrad_complete": "test-nb-energrad.out and test-nb-energrad-plateau.out (10 poses: energies, "
+                            "-dE/dM translation rows and torque blocks) on the mono-leu system",
     "energrad": "dE/dM = sum_j xh_j (x) g_j closed form; exact mode vs FP64 central differences",
     "optax_minimize": "adam: textbook Kingma & Ba steps and the constant-gradient property "
                       "(each step moves -lr g/(|g|+eps)); adabelief: constant-gradient closed form of "
                       "the belief recurrence; gravity: first-step closed form.  Against optax itself "
                       "parity unpinned (optax absent, version unpinned)",
-    "rmsd": "closed form",
+    "rmsd": "closed form; test-rmsd.lrmsd (mono-leu poses, 3 decimals)",
     "read_grid": "golden vectors from the reference reader on a synthetic grid file",
     "build_grid": "brute force over receptor beads at sampled grid points",
 }
@@ -288,6 +292,58 @@ def energrad(mat, coor_lig, gtypes, lig_fftypes, coor_rec, rec_fftypes, ff, grid
     dEdM = np.zeros((N, 4, 4))
     dEdM[:, :, :3] = np.einsum("jk,ijl->ikl", xh, g)
     return E, dEdM
+
+
+def nb_energrad_complete(mat, coor_lig, lig_fftypes, coor_rec, rec_fftypes, ff, origin, spacing, dim, plateaudis,
+                         plateau=False, exact=False):
+    """Neighbour term of energrad for a grid whose voxel lists are complete: every list holds
+    each receptor bead within plateaudis + half the voxel diagonal of its grid point (ATTRACT
+    lists built with neighbourdis at least that).  The list walk of nb_energy_single
+    (test-nb-energrad.py:148-158; plateau variant test-grid-energrad.py:129-159) then evaluates
+    exactly the pairs (bead inside the inner grid, receptor bead with dsq < plateaudis^2), so
+    they are found here with a k-d tree instead of lists.  Returns energies (N,), dE/dM
+    (N,4,4) and the sum of |pair energy| per pose (the scale of an energy that is a sum of
+    terms of both signs)."""
+    from scipy.spatial import cKDTree
+    xp = transform(mat, coor_lig)
+    N, nl = xp.shape[:2]
+    coor_rec = np.asarray(coor_rec, np.float64)
+    tree = cKDTree(coor_rec)
+    lt = np.asarray(lig_fftypes)
+    rt = np.asarray(rec_fftypes)
+    P2 = plateaudis ** 2
+    pE, pG = plateau_tables(ff, plateaudis)
+    E = np.zeros(N)
+    S = np.zeros(N)
+    g = np.zeros((N, nl, 3))
+    for i in range(N):
+        _, inside = voxelize(xp[i], origin, spacing, dim)
+        idx = np.nonzero(inside)[0]
+        lists = tree.query_ball_point(xp[i][idx], plateaudis)
+        c = np.array([len(l) for l in lists], np.int64)
+        if c.sum() == 0:
+            continue
+        j = np.repeat(idx, c)
+        r = np.concatenate([np.asarray(l, np.int64) for l in lists])
+        d = xp[i][j] - coor_rec[r]
+        dsq = (d * d).sum(1)
+        keep = dsq < P2
+        j, r, d, dsq = j[keep], r[keep], d[keep], dsq[keep]
+        at1, at2 = rt[r], lt[j]
+        prm = [np.asarray(a, np.float64)[at1, at2] for a in ff]
+        e = nonbon(dsq, *prm)
+        gs = 2 * nonbon_dsq_grad(dsq, *prm)
+        if plateau:
+            e = e - pE[at1, at2]
+            if not exact:
+                gs = gs - 2 * pG[at1, at2] * np.sqrt(P2 / dsq)
+        E[i] = e.sum()
+        S[i] = np.abs(e).sum()
+        np.add.at(g[i], j, gs[:, None] * d)
+    xh = np.concatenate([np.asarray(coor_lig, np.float64), np.ones((nl, 1))], axis=1)
+    dEdM = np.zeros((N, 4, 4))
+    dEdM[:, :, :3] = np.einsum("jk,ijl->ikl", xh, g)
+    return E, dEdM, S
 
 
 def dof_gradient(mat, dEdM):

@@ -13,9 +13,8 @@ needed at test time).
                        minim-sorted.dat, their Euler angles and ATTRACT's Energy / Gradients
                        lines (mono-leu/minim-sorted-grid.score); poses without a .dat entry
                        are left out
-  (test-rmsd.lrmsd is not used: test-rmsd.py:7 asserts equal atom counts, but the arrays
-  it loads hold 1275 and 1244 atoms, so its inputs cannot be reconstructed from the files
-  the reference holds; RMSD is pinned by closed forms instead)
+  (test-rmsd.lrmsd is pinned by make_mono_leu_fixture.py: its reference ligand is read from
+  mono-leu/minim-best-ligand.pdb, whose parsed .npy lost 31 records)
   pdb_excerpt.pdb      first 40 ATOM records of mono-leu/mono-leuc.pdb, with
   pdb_expected.npz     their coordinates and names from mono-leu/mono-leuc-pdb.npy
 """
