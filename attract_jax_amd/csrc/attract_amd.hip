@@ -728,7 +728,9 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
   uA = uB = 0.f;
 
-  for (int c0 = 0; c0 < A.nb; c0 += kWave) {
+  // a pose without beads has energy and gradient 0; paired with B it still runs B's chunk
+  const int nloop = max(A.nb, nposes == 2 ? B.nb : 0);
+  for (int c0 = 0; c0 < nloop; c0 += kWave) {
     const int j = halves ? lane & (kHalf - 1) : c0 + lane;
     const int nbh = h ? B.nb : A.nb;
     int cnt = 0, start = 0;
@@ -1504,12 +1506,14 @@ __global__ void fill_neighbours_kernel(GridView G, const float4* rec, int nrec, 
 // min(n + 1, ceil((d - r0) / h)) (h = shell width); groups in order, receptor order kept
 // within a group.  Count k of nbshell[v] = the entries in groups 0..k + 1: every entry
 // closer than r0 + (k + 1) h (the quotient rounds monotonically, so no such entry lands in
-// a later group).
+// a later group).  An entry that is not a bead of this receptor (r >= nrec: lists made for
+// another receptor) goes to the last group, which no shell reads; the pair kernels read it
+// as the far-away sentinel bead when they walk whole lists.
 __device__ __forceinline__ int shell_group(float d, float r0, float h, int n) {
   return d < r0 ? 0 : min(n + 1, (int)ceilf((d - r0) / h));
 }
 
-__global__ void shell_lists_kernel(GridView G, const float4* rec, int n, float r0, float h, int npts,
+__global__ void shell_lists_kernel(GridView G, const float4* rec, int nrec, int n, float r0, float h, int npts,
                                    uint16_t* out, int4* nbshell) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= npts) return;
@@ -1523,9 +1527,13 @@ __global__ void shell_lists_kernel(GridView G, const float4* rec, int n, float r
   for (int j = 0; j <= n + 1; ++j) {
     for (int t = 0; t < pl.x; ++t) {
       const uint16_t r = G.nblist[pl.y + t];
-      const float4 rb = rec[r];
-      const float dx = rb.x - px, dy = rb.y - py, dz = rb.z - pz;
-      if (shell_group(sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), r0, h, n) == j) out[pl.y + w++] = r;
+      int grp = n + 1;
+      if (r < nrec) {
+        const float4 rb = rec[r];
+        const float dx = rb.x - px, dy = rb.y - py, dz = rb.z - pz;
+        grp = shell_group(sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), r0, h, n);
+      }
+      if (grp == j) out[pl.y + w++] = r;
     }
     const int k = j - 1;  // shell k reads groups 0..k + 1
     if (k >= 0 && k < n) c[k >> 1] |= (unsigned)w << (16 * (k & 1));
@@ -1934,8 +1942,8 @@ int atj_grid_shell_lists(const AtjGrid* grid, const AtjReceptor* rec, int32_t ns
   const double h = (double)grid->spacing * std::sqrt(3.0) / (2.0 * nshell);
   const float r0 = (float)((double)grid->plateaudis + kShellMargin);
   hipLaunchKernelGGL(shell_lists_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream, G,
-                     reinterpret_cast<const float4*>(rec->beads), (int)nshell, r0, (float)h, (int)npts, nblist_out,
-                     reinterpret_cast<int4*>(nbshell_out));
+                     reinterpret_cast<const float4*>(rec->beads), rec->nrec, (int)nshell, r0, (float)h, (int)npts,
+                     nblist_out, reinterpret_cast<int4*>(nbshell_out));
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }

@@ -86,6 +86,8 @@ class Ligand:
         every bead; lig_atomtypes = pair-table column of every bead."""
         gt = np.asarray(lig_atomtype_pos, np.int64)
         lt = np.asarray(lig_atomtypes, np.int64)
+        if np.asarray(coor_lig).size == 0:
+            raise ValueError("a ligand needs at least one bead")
         if gt.min(initial=0) < 0 or gt.max(initial=0) > 0xFFFF or lt.min(initial=0) < 0 or lt.max(initial=0) > 0x7FFF:
             raise ValueError("ligand types out of range")
         b = pack_beads(coor_lig, gt | (lt << 16))

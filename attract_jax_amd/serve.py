@@ -99,6 +99,12 @@ class SlotPool:
                 merged.append((s, l))
         self.free = merged
 
+    def tail_free(self):
+        """Length of the free range that ends at the capacity (0 if the last slot is used)."""
+        if self.free and self.free[-1][0] + self.free[-1][1] == self.capacity:
+            return self.free[-1][1]
+        return 0
+
     def grow(self, new_capacity):
         if new_capacity <= self.capacity:
             return
@@ -152,6 +158,11 @@ class Scheduler:
     def _group_bytes(self, capacity):
         return capacity * (POSE_STATE_BYTES + PLAN_BYTES)
 
+    def _drop(self, g):
+        self.backend.drop_group(g)
+        self.used -= g.grid_bytes + self._group_bytes(g.pool.capacity)
+        del self.groups[g.key]
+
     def _evict_for(self, need, keep=None):
         """Free idle resident grids (LRU first) until `need` more bytes fit."""
         for key in list(self.groups):
@@ -160,47 +171,65 @@ class Scheduler:
             g = self.groups[key]
             if g is keep or g.jobs or g.inflight is not None:
                 continue
-            self.backend.drop_group(g)
-            self.used -= g.grid_bytes + self._group_bytes(g.pool.capacity)
-            del self.groups[key]
+            self._drop(g)
         return self.used + need <= self.budget
 
+    def _fail(self, job, msg):
+        job.state, job.error = "failed", msg
+        job.t_done = self.clock()
+        self.done.append(job)
+
     def _admit(self, now):
+        """Admit pending jobs in arrival order while they fit.  A job that would not fit even
+        with the device to itself fails at once; one that fits later waits (and holds back
+        the jobs behind it).  A group's slot pool grows by what the job lacks at its end; an
+        idle group too small for the job is dropped and made again at the job's size."""
         while self.pending:
             job = self.pending[0]
             key = job.spec.receptor_key
+            gbytes = self.backend.grid_bytes(key)
+            if gbytes + self._group_bytes(max(self.initial_slots, job.npose)) > self.budget:
+                self.pending.popleft()
+                self._fail(job, "job does not fit the HBM budget")
+                continue
             g = self.groups.get(key)
-            if g is None:
-                cap = max(self.initial_slots, job.npose)
-                need = self.backend.grid_bytes(key) + self._group_bytes(cap)
-                if need > self.budget:
-                    self.pending.popleft()
-                    job.state, job.error = "failed", "job does not fit the HBM budget"
-                    self.done.append(job)
-                    continue
-                if not self._evict_for(need):
-                    return  # wait until running jobs free memory
-                g = Group(key, self.backend.grid_bytes(key), SlotPool(cap))
-                self.backend.make_group(g)
-                self.used += need
-                self.groups[key] = g
+            start = -1
+            if g is not None:
                 start = g.pool.alloc(job.npose)
-            else:
-                start = g.pool.alloc(job.npose)
-                if start < 0:
-                    extra = max(job.npose, g.pool.capacity)
-                    need = self._group_bytes(extra)
-                    if g.inflight is not None or not self._evict_for(need, keep=g):
+                if start < 0 and not g.jobs and g.inflight is None:
+                    self._drop(g)
+                    g = None
+                elif start < 0:
+                    if g.inflight is not None:
                         return  # slot arrays are resized only between chunks
+                    extra = job.npose - g.pool.tail_free()
+                    need = self._group_bytes(extra)
+                    if not self._evict_for(need, keep=g):
+                        return  # wait until running jobs free memory
                     self.backend.grow_group(g, g.pool.capacity + extra)
                     g.pool.grow(g.pool.capacity + extra)
                     self.used += need
                     start = g.pool.alloc(job.npose)
+            if g is None:
+                cap = max(self.initial_slots, job.npose)
+                need = gbytes + self._group_bytes(cap)
+                if not self._evict_for(need):
+                    return  # wait until running jobs free memory
+                g = Group(key, gbytes, SlotPool(cap))
+                self.backend.make_group(g)
+                self.used += need
+                self.groups[key] = g
+                start = g.pool.alloc(job.npose)
             self.pending.popleft()
+            try:
+                self.backend.load_job(g, job, start)
+            except ValueError as e:  # a ligand the grid or pair table cannot score
+                g.pool.release(start, job.npose)
+                self._fail(job, str(e))
+                continue
             job.slot = start
             job.state = "admitted"
             job.t_admit = now
-            self.backend.load_job(g, job)
             g.jobs.append(job)
             g.dirty = True
             self.groups.move_to_end(g.key)
@@ -302,14 +331,17 @@ class GpuBackend:
         system = self._system(g.key)
         system.launch = _lib.AtjLaunch(self.waves, 0, 0)
         cap = g.pool.capacity
-        z = torch.zeros
-        state = PoseState(z((cap, 7), device=self.device), z((cap, 7), device=self.device),
-                          z((cap, 7), device=self.device), z(cap, dtype=torch.int32, device=self.device),
-                          z(cap, device=self.device))
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
         stream.wait_stream(torch.cuda.current_stream(self.device))  # grid built on the current stream
-        counter = torch.zeros(1, dtype=torch.int32, device=self.device)
+        # the group's arrays belong to its stream: the caching allocator then reuses their
+        # memory only in that stream's order once they are dropped (grow_group, drop_group)
+        z = torch.zeros
+        with torch.cuda.stream(stream):
+            state = PoseState(z((cap, 7), device=self.device), z((cap, 7), device=self.device),
+                              z((cap, 7), device=self.device), z(cap, dtype=torch.int32, device=self.device),
+                              z(cap, device=self.device))
+            counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         g.handle = dict(system=system, state=state, stream=stream, counter=counter, plan=None)
 
     def grow_group(self, g, new_capacity):
@@ -331,13 +363,16 @@ class GpuBackend:
         g.handle = None
         self.torch.cuda.empty_cache()
 
-    def load_job(self, g, job):
+    def load_job(self, g, job, slot):
+        """Device ligand and starting poses of a job in slots [slot, slot + npose).  ValueError
+        (nothing loaded) for a ligand without beads or with types beyond the grid or table."""
         from .device import Ligand
         torch = self.torch
         h = g.handle
         sp = job.spec
         lig = Ligand.create(sp.ligand_coor, sp.ligand_types - 1, sp.ligand_types - 1, self.device)
-        sl = slice(job.slot, job.slot + job.npose)
+        lig.check_types(h["system"].grid, h["system"].ff)
+        sl = slice(slot, slot + job.npose)
         with torch.cuda.stream(h["stream"]):
             pose = torch.cat([torch.as_tensor(sp.quat, dtype=torch.float32),
                               torch.as_tensor(sp.trans, dtype=torch.float32)], 1).pin_memory()
@@ -404,12 +439,14 @@ class GpuBackend:
         job.handle = None
 
 
-def run_stream(scheduler, specs, poll=0.0001, timeout=600.0):
+def run_stream(scheduler, specs, poll=0.0001, timeout=600.0, stall=60.0):
     """Submit specs at their arrival times (seconds from now) and run until all are done.
-    Returns (latencies (s), makespan (s), pose-steps)."""
+    Returns (latencies (s), makespan (s), pose-steps).  Raises RuntimeError when jobs wait
+    or run but none advances for `stall` seconds, TimeoutError after `timeout`."""
     t0 = scheduler.clock()
     order = sorted(specs, key=lambda s: s.arrival)
     k = 0
+    mark, t_mark = None, t0
     while True:
         now = scheduler.clock()
         while k < len(order) and now - t0 >= order[k].arrival:
@@ -419,6 +456,13 @@ def run_stream(scheduler, specs, poll=0.0001, timeout=600.0):
         scheduler.tick(now)
         if k == len(order) and scheduler.idle():
             break
+        progress = (len(scheduler.done), sum(j.steps_done for j in scheduler.jobs))
+        if progress != mark or scheduler.idle():
+            mark, t_mark = progress, now
+        elif now - t_mark > stall:
+            waiting = [j.id for j in scheduler.pending]
+            raise RuntimeError(f"no job advanced for {stall:.0f} s (pending jobs {waiting[:8]}, "
+                               f"{len(scheduler.jobs) - len(scheduler.done)} unfinished)")
         if now - t0 > timeout:
             raise TimeoutError("serving run exceeded its time limit")
         time.sleep(poll)

@@ -481,3 +481,36 @@ def test_grid_term_only(s1):
     xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
     r = O.rmsd(xg, xt)
     assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
+
+
+def test_pose_without_beads_next_to_a_small_one(s1):
+    """A job whose ligand has no beads (a C caller can pass one) beside a job of at most 32
+    beads: the empty poses get energy and gradient 0, and the small poses, which share a
+    bead phase with them, get the bits they get alone; in the minimizer too."""
+    from attract_jax_amd.device import Ligand
+    system, lig = s1.system, s1.ligs[0]
+    empty = Ligand(lig.beads, 0)
+    n = 64
+    mat = s1.mats(slice(0, n))
+    jobs = system.jobs_tensor([empty, lig])
+    pose_job = torch.from_numpy((np.arange(n) % 2).astype(np.int32)).to(DEV)
+    E, G = system.energrad(mat, None, pose_job=pose_job, jobs=jobs, njobs=2)
+    Ea, Ga = system.energrad(mat[1::2], lig)
+    assert torch.count_nonzero(E[0::2]) == 0 and torch.count_nonzero(G[0::2]) == 0
+    assert torch.equal(E[1::2], Ea) and torch.equal(G[1::2], Ga)
+
+    w = s1.w
+    half = n // 2
+    q, t = w.quat[:n], w.trans[:n]
+    plan = BatchPlan(system, [empty, lig], [half, half], 20, w.lr_rot, w.lr_trans)
+    inter = np.stack([np.arange(half), half + np.arange(half)], 1).reshape(-1).astype(np.int32)
+    plan.order = torch.from_numpy(inter).to(DEV)
+    st = PoseState.create(q, t, DEV)
+    minimize(system, plan, st, 20)
+    alone = BatchPlan(system, [lig], [half], 20, w.lr_rot, w.lr_trans)
+    sa = PoseState.create(q[half:], t[half:], DEV)
+    minimize(system, alone, sa, 20)
+    torch.cuda.synchronize()
+    assert torch.equal(st.pose[half:], sa.pose) and torch.equal(st.energy[half:], sa.energy)
+    assert torch.count_nonzero(st.energy[:half]) == 0
+    assert torch.equal(st.pose[:half, 4:], torch.as_tensor(t[:half], dtype=torch.float32, device=DEV))

@@ -34,7 +34,7 @@ class FakeBackend:
     def drop_group(self, g):
         self.dropped.append(g.key)
 
-    def load_job(self, g, job):
+    def load_job(self, g, job, slot):
         job.handle = {}
 
     def launch(self, g, jobs, steps, dirty):
@@ -181,3 +181,89 @@ def test_long_job_is_not_starved():
     assert a.steps_done > 0  # advanced while short jobs kept coming
     drive(s, clock)
     assert a.state == "done" and all(j.state == "done" for j in shorts)
+
+
+def test_idle_group_too_small_is_made_again_at_the_job_size():
+    """Grid 100 B, 100 B per slot, 16 initial slots, budget 5000 B: after a 10-pose job the
+    idle 16-slot group cannot grow by 40 (5700 B) but a new 40-slot group fits (4100 B)."""
+    clock = FakeClock()
+    be = FakeBackend(clock, {0: 100})
+    s = Scheduler(be, hbm_budget=5000, chunk_steps=10, initial_slots=16, clock=clock)
+    a = s.submit(spec(0, 10, 20))
+    drive(s, clock)
+    b = s.submit(spec(0, 40, 20))
+    drive(s, clock, limit=1000)
+    assert a.state == b.state == "done"
+    assert s.groups[0].pool.capacity == 40 and s.used <= 5000
+
+
+def test_pool_grows_by_what_the_job_lacks():
+    clock = FakeClock()
+    be = FakeBackend(clock, {0: 100})
+    s = Scheduler(be, hbm_budget=10 ** 6, chunk_steps=10, initial_slots=16, clock=clock)
+    a = s.submit(spec(0, 10, 200))
+    s.tick()
+    b = s.submit(spec(0, 20, 20))  # 6 slots free at the end: grows by 14
+    for _ in range(5):
+        clock.t += 1.0
+        s.tick()
+    assert b.state in ("admitted", "done") and s.groups[0].pool.capacity == 30
+    drive(s, clock)
+    assert a.state == b.state == "done"
+
+
+class TickClock:
+    """A clock that advances by dt at every reading (run_stream reads it in its loop)."""
+
+    def __init__(self, dt=1e-4):
+        self.t, self.dt = 0.0, dt
+
+    def __call__(self):
+        self.t += self.dt
+        return self.t
+
+
+def test_run_stream_twice_over_the_same_specs():
+    """A spec's arrival stays relative to the start of the stream: a second run over the same
+    list submits and completes every job again (the scheduler records its own clock)."""
+    from attract_jax_amd.serve import run_stream
+    specs = [spec(i % 2, 20 + i, 15, arrival=0.002 * i) for i in range(6)]
+    before = [sp.arrival for sp in specs]
+    for _ in range(2):
+        clock = TickClock()
+        be = FakeBackend(clock, {0: 1000, 1: 1000})
+        s = Scheduler(be, hbm_budget=10 ** 9, chunk_steps=5, initial_slots=64, clock=clock)
+        lat, makespan, ps = run_stream(s, specs, poll=0.0, timeout=50.0)
+        assert len(lat) == len(specs) and all(j.state == "done" for j in s.jobs)
+        assert ps == sum(len(sp.quat) * sp.nsteps for sp in specs)
+    assert [sp.arrival for sp in specs] == before
+
+
+def test_run_stream_raises_when_nothing_advances():
+    from attract_jax_amd.serve import run_stream
+
+    class Stuck(FakeBackend):
+        def finished(self, token):
+            return False
+
+    clock = TickClock(dt=0.01)
+    s = Scheduler(Stuck(clock, {0: 1000}), hbm_budget=10 ** 9, chunk_steps=5, initial_slots=64, clock=clock)
+    import pytest
+    with pytest.raises(RuntimeError, match="no job advanced"):
+        run_stream(s, [spec(0, 10, 10)], poll=0.0, timeout=100.0, stall=1.0)
+
+
+def test_job_the_backend_cannot_load_fails_without_taking_slots():
+    class Picky(FakeBackend):
+        def load_job(self, g, job, slot):
+            if len(job.spec.ligand_coor) == 0:
+                raise ValueError("a ligand needs at least one bead")
+            job.handle = {}
+
+    clock = FakeClock()
+    s = Scheduler(Picky(clock, {0: 100}), hbm_budget=10 ** 6, chunk_steps=10, initial_slots=16, clock=clock)
+    bad = s.submit(JobSpec(0, np.zeros((0, 3)), np.ones(0, int), np.zeros((5, 4)), np.zeros((5, 3)), 10))
+    good = s.submit(spec(0, 16, 10))
+    drive(s, clock)
+    assert bad.state == "failed" and "bead" in bad.error
+    assert good.state == "done" and s.groups[0].pool.capacity == 16
