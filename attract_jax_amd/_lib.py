@@ -128,6 +128,9 @@ SIGNATURES = {
     "atj_minimize": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjForceField), ctypes.POINTER(AtjReceptor),
                       _P, _I, _P, _P, _I, ctypes.POINTER(AtjPoseState), _I, _U, ctypes.POINTER(AtjLaunch),
                       _P, _P, _P], ctypes.c_int),
+    "atj_minimize_noalloc": ([ctypes.POINTER(AtjGrid), ctypes.POINTER(AtjForceField), ctypes.POINTER(AtjReceptor),
+                              _P, _I, _P, _P, _I, ctypes.POINTER(AtjPoseState), _I, _U, ctypes.POINTER(AtjLaunch),
+                              _P, _P], ctypes.c_int),
     "atj_ligand_rmsd": ([_P, _I, _P, _P, _I, _P, _P], ctypes.c_int),
     "atj_optimizer_step": ([_P, _P, _I, ctypes.POINTER(AtjPoseState), ctypes.POINTER(AtjOptimizer), _P],
                            ctypes.c_int),

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "attract_amd.h"
@@ -1757,6 +1758,35 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
 }
 
+// Scratch memory of the allocation-free entry point (atj_minimize_noalloc): the work
+// counter of its persistent launch lives here instead of in a caller workspace.  One copy per
+// GPU, held until the process ends.  Host code never uses the array's name as an address:
+// each device's address comes from hipGetSymbolAddress on that device's first call.  Every
+// call on a GPU shares that GPU's copy, so calls that overlap on one GPU (on different
+// streams) race on the counter.
+constexpr size_t kScratchBytes = 256;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) int scratch[kScratchBytes / sizeof(int)];
+
+std::mutex g_scratch_mu;
+int* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the scratch array on the current device (looked up once per device).
+int scratch_address(int** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ATJ_ERR_UNSUPPORTED, "device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ATJ_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_dev[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return ATJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1811,6 +1841,19 @@ int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor
   a.counter = counter;
   a.stats = stats;
   return launch_pose_kernel<kMinimize>(a, launch, (hipStream_t)stream);
+}
+
+int atj_minimize_noalloc(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                         const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job, const int32_t* pose_order,
+                         int32_t N, const AtjPoseState* state, int32_t nsteps_launch, uint32_t flags,
+                         const AtjLaunch* launch, unsigned long long* stats, void* stream) {
+  int* counter = nullptr;
+  int rc = scratch_address(&counter);
+  if (rc) return rc;
+  // the work counter takes the first int of the scratch array; the launch resets it with a
+  // fill kernel (no memset node, so the call may be captured in a graph)
+  return atj_minimize(grid, ff, rec, jobs_dev, njobs, pose_job, pose_order, N, state, nsteps_launch, flags, launch,
+                      counter, stats, stream);
 }
 
 int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,

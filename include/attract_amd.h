@@ -206,6 +206,19 @@ int atj_minimize(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor
                  int32_t nsteps_launch, uint32_t flags, const AtjLaunch* launch,
                  int32_t* counter, unsigned long long* stats, void* stream);
 
+/* atj_minimize without the work-counter workspace: for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid hipMalloc).  No
+ * counterpart in the reference.  The counter lives in a 256-byte scratch array the library
+ * holds on each GPU until the process ends; the call allocates nothing and needs no size
+ * limit beyond atj_minimize's (the scratch does not grow with N).  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (different streams) race and
+ * must be serialised by the caller.  Arguments and results as atj_minimize. */
+int atj_minimize_noalloc(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
+                         const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
+                         const int32_t* pose_order, int32_t N, const AtjPoseState* state,
+                         int32_t nsteps_launch, uint32_t flags, const AtjLaunch* launch,
+                         unsigned long long* stats, void* stream);
+
 /* One optimizer step of N poses from dE/dM (the plain one-launch-per-step form):
  * quaternion chain rule through as_matrix, the gravity term (energy[p] += gravity |t|^2
  * when energy is not NULL), the optimizer update, renormalisation.  Same arithmetic as
