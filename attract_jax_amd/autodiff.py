@@ -72,24 +72,26 @@ class PoseEnergy(torch.autograd.Function):
         return grad_mat, g_rc, g_ac, None, g_emin, None, None, None, None, None
 
 
-def param_adjoint(system, ligand, M, ff, weights, flags, blocks=None, jobs=None):
-    """sum_pose w_pose dE_pose/d(rc, ac, emin) -> (3, nrt, nlt) float32 (neighbour-list term)."""
+def param_adjoint(system, ligand, M, ff, weights, flags, blocks=None, jobs=None, waves_per_block=None,
+                  options=None):
+    """sum_pose w_pose dE_pose/d(rc, ac, emin) -> (3, nrt, nlt) float32 (neighbour-list term).
+    Reproducible bit for bit for a given launch shape (blocks, waves per block)."""
     lib = _lib.load()
     d = system.device
     N = M.shape[0]
-    if blocks is None:
-        blocks = 2 * torch.cuda.get_device_properties(d).multi_processor_count
+    if blocks is None:  # one block per CU: a block's wavefront tables fill most of its LDS
+        blocks = torch.cuda.get_device_properties(d).multi_processor_count
     ntp = ff.nrt * ff.nlt
     partial = torch.empty((blocks, 3 * ntp), dtype=torch.float32, device=d)
     out = torch.empty(3 * ntp, dtype=torch.float32, device=d)
-    energy = torch.empty(N, dtype=torch.float32, device=d)
     if jobs is None:
         jobs = system.jobs_tensor([ligand])
-    launch = _lib.AtjLaunch(system.launch.waves_per_block, blocks, system.launch.options)
+    launch = _lib.AtjLaunch(waves_per_block or 12, blocks,
+                            system.launch.options if options is None else options)
     _lib.check(lib.atj_energrad_adjoint(ctypes.byref(system._gs), ctypes.byref(ff.struct()),
                                         ctypes.byref(system._rs), _ptr(jobs), 1, ctypes.c_void_p(0), _ptr(M), N,
-                                        _ptr(weights), _ptr(energy), ctypes.c_void_p(0), _ptr(partial), flags,
-                                        ctypes.byref(launch), _ptr(system.counter), _stream(d)))
+                                        _ptr(weights), ctypes.c_void_p(0), ctypes.c_void_p(0), _ptr(partial), flags,
+                                        ctypes.byref(launch), ctypes.c_void_p(0), _stream(d)))
     _lib.check(lib.atj_reduce_partials(_ptr(partial), blocks, 3 * ntp, _ptr(out), _stream(d)))
     return out.reshape(3, ff.nrt, ff.nlt)
 

@@ -633,34 +633,37 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
-        float E, h;
-        pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, h);
-        e += E;
-        gx = fmaf(h, dd[cc][0], gx);
-        gy = fmaf(h, dd[cc][1], gy);
-        gz = fmaf(h, dd[cc][2], gz);
         npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
-        if (MODE == kAdjoint && ok[cc]) {
-          float dp[3];
-          pair_param_grad(dsq[cc], __builtin_amdgcn_rcpf(dsq[cc]), pa[cc], dp);
-          if (PLATEAU) {
-            float dq[3];
-            pair_param_grad(P2, uP2, pa[cc], dq);
-            dp[0] -= dq[0];
-            dp[1] -= dq[1];
-            dp[2] -= dq[2];
+        if (MODE == kAdjoint) {
+          // only the parameter derivatives, added to this wavefront's own table: no other
+          // wavefront writes it, and its adds come in a fixed order, so sums are reproducible
+          if (ok[cc]) {
+            float dp[3];
+            pair_param_grad(dsq[cc], __builtin_amdgcn_rcpf(dsq[cc]), pa[cc], dp);
+            if (PLATEAU) {
+              float dq[3];
+              pair_param_grad(P2, uP2, pa[cc], dq);
+              dp[0] -= dq[0];
+              dp[1] -= dq[1];
+              dp[2] -= dq[2];
+            }
+            const unsigned pg = pidx[cc];
+            atomicAdd(&s_pgrad[pg], wgt * dp[0]);
+            atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
+            // the emin derivative is zero except for repulsive pairs inside rmin
+            if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
           }
-          // the gradient rows have the pair table's odd stride, so lanes adding to one ligand
-          // type under different receptor types hit different LDS banks
-          const unsigned pg = pidx[cc];
-          atomicAdd(&s_pgrad[pg], wgt * dp[0]);
-          atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
-          // the emin derivative is zero except for repulsive pairs inside rmin: LDS float
-          // atomics are slow, so the zero ones are not issued
-          if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
+        } else {
+          float E, h;
+          pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, h);
+          e += E;
+          gx = fmaf(h, dd[cc][0], gx);
+          gy = fmaf(h, dd[cc][1], gy);
+          gz = fmaf(h, dd[cc][2], gz);
         }
       }
     }
+    if (MODE == kAdjoint) continue;
     // the gradient is -2 (gx, gy, gz); bl holds -2 x_local, and scaling by -2 is exact, so
     // fma(-2 x, g', acc) and fma(-2, g', acc) equal the products with the scaled gradient
     acc[0] += e;
@@ -760,7 +763,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       int voxel;
       unsigned in_any;
       bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
-      bead_acc(acc, Eg, g, xl);
+      if (MODE != kAdjoint) bead_acc(acc, Eg, g, xl);
       st[0] += in_any;
       if (voxel >= 0) {
         st[1] += 1;
@@ -799,7 +802,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     wave_sync();
     unsigned npair = 0;  // wave-wide count, credited to lane 0
     for (int q = 0; q < nposes; ++q) {
-      if (q == 1) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
+      if (q == 1 && MODE != kAdjoint) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
         uA = wave_sum_all(acc, lane);
 #pragma unroll
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
@@ -843,6 +846,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     st[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
+  if (MODE == kAdjoint) return;  // the parameter derivatives went to the wavefront's table
   const float u = wave_sum_all(acc, lane);
   if (nposes == 2)
     uB = u;
@@ -982,7 +986,7 @@ __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float 
 // Stage the receptor beads (+ the padding sentinel) and the pair table (+ the
 // all-zero record) into LDS.
 __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* s_side, bool plateau, bool exact,
-                          float* s_pgrad) {
+                          float* s_pgrad, int npgrad) {
   const int tid = threadIdx.x;
   if (s_rec) {
     for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = clamp_rec_type(a.rec[i], a.nrt);
@@ -1003,7 +1007,7 @@ __device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* 
     s_side[i] = t.side;
   }
   if (s_pgrad)
-    for (int i = tid; i < 3 * nslots; i += blockDim.x) s_pgrad[i] = 0.f;
+    for (int i = tid; i < npgrad; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
 // Work claims of a persistent wavefront, `a.batch` poses each.  In the blocks the device holds
@@ -1175,10 +1179,12 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_mat = reinterpret_cast<float*>(s_ws + nwaves);
+  // adjoint: one pair-table gradient [3][nslots] per wavefront
   float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * (kBatch * 12) : nullptr;
-  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad);
+  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad, nwaves * 3 * nslots);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
+  float* w_pgrad = MODE == kAdjoint ? s_pgrad + wave * 3 * nslots : nullptr;
   unsigned st[4] = {0, 0, 0, 0};
 
   if (MODE == kMinimize) {
@@ -1190,7 +1196,12 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
     // for the whole claim at once (one round of memory latency per claim, not per pose).
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
     float* Ms = s_mat + wave * (kBatch * 12);
-    for (long long claim = first_claim(a, lane); claim < a.N; claim = next_claim(a, lane)) {
+    // the adjoint takes its claims in a fixed order (wavefront g of the grid: claims g, g + G,
+    // g + 2G, ...), so each wavefront's table sums the same poses in the same order every run
+    const long long wave_id = (long long)blockIdx.x * nwaves + wave;
+    const long long stride_claims = (long long)gridDim.x * nwaves * a.batch;
+    for (long long claim = MODE == kAdjoint ? wave_id * a.batch : first_claim(a, lane); claim < a.N;
+         claim = MODE == kAdjoint ? claim + stride_claims : next_claim(a, lane)) {
       const int k0 = (int)claim;
       const int nbatch = min(a.batch, a.N - k0);
       int p_l = 0;
@@ -1231,13 +1242,13 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
           }
         }
         float u[2];
-        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_pgrad, ws, P[0], P[1], st,
+        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, w_pgrad, ws, P[0], P[1], st,
                                                              lane, u[0], u[1]);
-        write_energrad(a, p[0], u[0], lane);
-        if (p[1] >= 0) {
-          write_energrad(a, p[1], u[1], lane);
-          ++b;
+        if (MODE != kAdjoint) {
+          write_energrad(a, p[0], u[0], lane);
+          if (p[1] >= 0) write_energrad(a, p[1], u[1], lane);
         }
+        if (p[1] >= 0) ++b;
       }
     }
   }
@@ -1248,13 +1259,16 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
       if (lane == 0 && s) atomicAdd(a.stats + i, (unsigned long long)s);
     }
   }
-  if (MODE == kAdjoint) {
+  if (MODE == kAdjoint) {  // the block's row of partial sums: wavefront tables added in order
     __syncthreads();
     float* out = a.partial + (size_t)blockIdx.x * 3 * ntp;
-    const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
+    const int stride = pair_stride(a.nlt);
     for (int i = threadIdx.x; i < 3 * ntp; i += blockDim.x) {
       const int c = i / ntp, k = i - c * ntp, rt = k / a.nlt;
-      out[i] = s_pgrad[c * nslots + rt * stride + (k - rt * a.nlt)];
+      const int idx = c * nslots + rt * stride + (k - rt * a.nlt);
+      float v = s_pgrad[idx];
+      for (int w = 1; w < nwaves; ++w) v += s_pgrad[w * 3 * nslots + idx];
+      out[i] = v;
     }
   }
 }
@@ -1281,6 +1295,11 @@ __global__ void ligand_rmsd_kernel(const float* __restrict__ poses, int N, const
 }
 
 __global__ void fill_int(int* p, int v) { *p = v; }
+
+__global__ void zero_floats(float* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
 
 // Per-(pose, bead) terms; one thread per bead-instance (parity / debugging path).
 // With coords != NULL the items are explicit points (coords[idx], types[idx]) instead
@@ -1709,8 +1728,10 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
   a.static_blocks = min(blocks, resident);
-  hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
-  HIPCHK(hipGetLastError());
+  if (MODE != kAdjoint) {  // the adjoint's claims are static: no work counter
+    hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
+    HIPCHK(hipGetLastError());
+  }
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(wpb * kWave), lds, stream, a);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
@@ -1725,7 +1746,10 @@ int launch_layout(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, h
 
 template <int MODE, bool REC_LDS>
 int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hipStream_t stream) {
-  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0, exact = (a.flags & ATJ_GRAD_EXACT) != 0;
+  // the adjoint's parameter derivatives are those of the energy: the gradient convention
+  // (exact or force-shifted) does not enter them
+  const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0;
+  const bool exact = MODE != kAdjoint && (a.flags & ATJ_GRAD_EXACT) != 0;
   if (plateau && exact) return launch_layout<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
   if (plateau) return launch_layout<MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
   if (exact) return launch_layout<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
@@ -1734,7 +1758,7 @@ int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hi
 
 template <int MODE>
 int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
-  if (!a.counter) return fail(ATJ_ERR_INVALID, "work counter is NULL");
+  if (!a.counter && MODE != kAdjoint) return fail(ATJ_ERR_INVALID, "work counter is NULL");
   if (!a.jobs) return fail(ATJ_ERR_INVALID, "jobs is NULL");
   DevInfo d;
   int rc = device_info(d);
@@ -1745,14 +1769,31 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   const uint32_t options = launch ? launch->options : 0u;
   if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
   const int nslots = pair_slots(a.nrt, a.nlt);
-  const size_t base = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2) +
-                      wpb * sizeof(WaveScratch) +
-                      (MODE == kMinimize ? wpb * sizeof(BatchScratch) : wpb * kBatch * 12 * sizeof(float)) +
-                      (MODE == kAdjoint ? 3 * (size_t)nslots * 4 : 0);
+  const size_t fixed = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2);
+  const size_t per_wave = sizeof(WaveScratch) +
+                          (MODE == kMinimize ? sizeof(BatchScratch) : kBatch * 12 * sizeof(float)) +
+                          (MODE == kAdjoint ? 3 * (size_t)nslots * sizeof(float) : 0);
+  const size_t rec_bytes = (size_t)(a.nrec + 1) * sizeof(float4);
+  bool rec_lds = !(options & ATJ_LAUNCH_REC_GLOBAL);
+  if (MODE == kAdjoint) {
+    // every wavefront holds its own gradient table: as many wavefronts as fit beside the pair
+    // table, and the receptor in LDS only when it costs no wavefront (C4: 8 wavefronts with
+    // the receptor read through L1 take 0.45 ms, 6 with it in LDS 0.48 ms)
+    int w_lds = wpb, w_glob = wpb;
+    while (w_lds > 1 && fixed + w_lds * per_wave + rec_bytes > d.lds_per_block) --w_lds;
+    while (w_glob > 1 && fixed + w_glob * per_wave > d.lds_per_block) --w_glob;
+    if (rec_lds && w_lds == w_glob && fixed + w_lds * per_wave + rec_bytes <= d.lds_per_block) {
+      wpb = w_lds;
+    } else {
+      rec_lds = false;
+      wpb = w_glob;
+    }
+  }
+  const size_t base = fixed + wpb * per_wave;
   if (base > d.lds_per_block) return fail(ATJ_ERR_UNSUPPORTED, "pair table does not fit in LDS");
   const int blocks = launch && launch->blocks > 0 ? launch->blocks : 0;
-  const size_t with_rec = base + (size_t)(a.nrec + 1) * sizeof(float4);
-  if (!(options & ATJ_LAUNCH_REC_GLOBAL) && with_rec <= d.lds_per_block)
+  const size_t with_rec = base + rec_bytes;
+  if (rec_lds && with_rec <= d.lds_per_block)
     return launch_flags<MODE, true>(a, wpb, with_rec, blocks, d, stream);
   if (a.nrec == 0) return fail(ATJ_ERR_INVALID, "global receptor path needs at least one bead");
   return launch_flags<MODE, false>(a, wpb, base, blocks, d, stream);
@@ -1865,17 +1906,35 @@ int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const Atj
   if (rc) return rc;
   if (N < 0 || njobs < 1) return fail(ATJ_ERR_INVALID, "N < 0 or no job");
   if (!launch || launch->blocks <= 0) return fail(ATJ_ERR_INVALID, "adjoint needs launch->blocks > 0");
-  if (!poses || !energy || !weights || !partial) return fail(ATJ_ERR_INVALID, "NULL argument");
+  if (!poses || !weights || !partial) return fail(ATJ_ERR_INVALID, "NULL argument");
+  if (dEdM && !energy) return fail(ATJ_ERR_INVALID, "dEdM needs energy");
   a.jobs = jobs_dev;
   a.pose_job = pose_job;
   a.N = N;
   a.poses = poses;
-  a.energy = energy;
-  a.dEdM = dEdM;
+  hipStream_t st = (hipStream_t)stream;
+  if (energy && N > 0) {  // energies and dE/dM: the energrad launch
+    KArgs e = a;
+    e.energy = energy;
+    e.dEdM = dEdM;
+    e.counter = counter;
+    AtjLaunch le = *launch;
+    le.blocks = 0;
+    rc = launch_pose_kernel<kEnergrad>(e, &le, st);
+    if (rc) return rc;
+  }
+  // the pair-table gradient: a pass of its own over the neighbour-list pairs (the potential
+  // grids do not depend on the pair table), with partial sums per block
+  a.flags = flags & ~(uint32_t)ATJ_TERM_GRID;
   a.weights = weights;
   a.partial = partial;
-  a.counter = counter;
-  return launch_pose_kernel<kAdjoint>(a, launch, (hipStream_t)stream);
+  if (!(a.flags & ATJ_TERM_NB) || N == 0) {
+    const long long n = (long long)launch->blocks * 3 * ff->nrt * ff->nlt;
+    hipLaunchKernelGGL(zero_floats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, n);
+    HIPCHK(hipGetLastError());
+    return ATJ_OK;
+  }
+  return launch_pose_kernel<kAdjoint>(a, launch, st);
 }
 
 int atj_reduce_partials(const float* partial, int32_t blocks, int32_t n, float* out, void* stream) {

@@ -234,9 +234,13 @@ int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float
 
 /* Adjoint of the pair term w.r.t. the pair table: for weights w[N]
  * (upstream dL/dE_pose) accumulates sum_pose w * dE_pose/d(rc, ac, emin) into
- * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated) and
- * also writes energy/dEdM like atj_energrad (recomputation, no stored pairs).
- * Reduce the slabs with atj_reduce_partials. Requires blocks = launch->blocks > 0. */
+ * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated), recomputing
+ * the pairs (nothing per pair is stored).  Reduce the slabs with atj_reduce_partials.
+ * Requires blocks = launch->blocks > 0.  The sums are reproducible bit for bit: each
+ * wavefront adds into a table of its own, takes its poses in a fixed order, and the tables
+ * and slabs are added in a fixed order.  energy (nullable) and dEdM (nullable, needs energy)
+ * receive what atj_energrad writes, by an energrad launch before the gradient pass (using
+ * counter); with both NULL only the gradient pass runs and counter may be NULL. */
 int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
                          const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
                          const float* poses, int32_t N, const float* weights, float* energy,

@@ -29,6 +29,23 @@ def _oracle_total(w, grid, ff, mat, weights):
     return float((weights * E).sum()), E
 
 
+def _smooth_poses(mat, coor_lig, coor_rec, grid, margin):
+    """Poses whose beads all lie more than `margin` A from every voxel face of the grid they are
+    looked up in, from the inner grid's boundary, and (for every receptor bead) from the
+    plateau sphere."""
+    xp = O.transform(mat, coor_lig)
+    v, inside = O.voxelize(xp.reshape(-1, 3), grid.origin, grid.gridspacing, grid.dim)
+    vo = (v + grid.gridextension) / 2
+    frac = np.where(inside[:, None], v, vo)
+    cell = np.where(inside[:, None], grid.gridspacing, 2 * grid.gridspacing)
+    face = (np.abs(frac - np.round(frac)) * cell).min(axis=1)
+    edge = np.minimum(np.abs(v), np.abs(v - (np.asarray(grid.dim) - 1))).min(axis=1) * grid.gridspacing
+    d = np.linalg.norm(xp[:, :, None, :] - np.asarray(coor_rec)[None, None], axis=-1)
+    shell = np.abs(d - grid.plateaudis).min(axis=2).reshape(-1)
+    ok = (face > margin) & (edge > margin) & (shell > margin)
+    return ok.reshape(len(mat), -1).all(axis=1)
+
+
 def test_param_gradient_vs_fd(sysc1):
     w, system, ligs, grid = sysc1
     mat = mat_from_qt(w.quat, w.trans)
@@ -89,13 +106,19 @@ def test_pose_dof_gradient_vs_fd(sysc1):
                         w.rec_types - 1, fff, grid, exact=True)[0]
         fd[:, a] = (Ep - Em) / (2 * h)
     scale = np.maximum(np.abs(fd).max(axis=1, keepdims=True), 1.0)
-    ok = np.all(np.abs(dof - fd) / scale < 2e-3, axis=1)
-    assert ok.mean() >= 0.9
+    err = np.abs(dof - fd) / scale
+    # the energy is smooth in the pose only between voxel faces, the inner/outer boundary and
+    # the plateau sphere: poses whose beads keep 2e-3 A from all of them (the perturbations
+    # move a bead by at most 1e-4 x its distance from the ligand centre) must all agree
+    smooth = _smooth_poses(mat, l.coor, w.coor_rec, grid, margin=2e-3)
+    assert smooth.sum() >= 32, smooth.sum()
+    assert err[smooth].max() < 2e-3, err[smooth].max()
+    assert (err < 2e-3).mean() >= 0.9
 
 
 def test_adjoint_deterministic_and_attract_mode(sysc1):
-    """Repeated adjoint launches agree to float32 rounding of the per-block sums; the ATTRACT
-    gradient convention leaves the parameter gradient of the energy unchanged."""
+    """Repeated adjoint launches give the same bits; the ATTRACT gradient convention leaves the
+    parameter gradient of the energy unchanged (the same bits too)."""
     w, system, ligs, grid = sysc1
     mat = mat_from_qt(w.quat, w.trans)
     mat[:, 3, :3] *= 0.55
@@ -107,6 +130,49 @@ def test_adjoint_deterministic_and_attract_mode(sysc1):
         E = pose_energy(M, rc, ff.ac, ff.ivor, ff.emin, ff.rmin2, system, ligs[0], flags=flags)
         E.sum().backward()
         outs.append(rc.grad.clone())
-    scale = outs[0].abs().max()
-    assert ((outs[0] - outs[1]).abs().max() / scale) < 1e-5
-    assert ((outs[0] - outs[2]).abs().max() / scale) < 1e-5
+    assert outs[0].abs().max() > 0
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
+def test_c4_param_gradient_vs_fd():
+    """The C4 system itself (2000-bead receptor, 96^3 + 64^3 grids, 21-bead ligand): the pair-table
+    gradient of a weighted sum over 64 poses through the autograd Function, w.r.t. (rmin, |emin|)
+    of the eight type pairs with the largest rmin gradient, against FP64 central differences
+    (h = 1e-4) of the oracle's neighbour-list energy (the potential grids are data)."""
+    from attract_jax_amd.workloads import c4
+    w = c4(npose=64)
+    system, ligs = build_system(w, DEV)
+    g = system.grid
+    mat = mat_from_qt(w.quat, w.trans)
+    mat[:, 3, :3] *= 0.8  # into contact with the receptor surface
+    rng = np.random.default_rng(3)
+    weights = rng.uniform(0.5, 1.5, len(mat))
+    rmin = torch.tensor(w.rmin, dtype=torch.float64, device=DEV, requires_grad=True)
+    eabs = torch.tensor(w.emin_abs, dtype=torch.float64, device=DEV, requires_grad=True)
+    ivor = torch.tensor(w.ivor, dtype=torch.float64, device=DEV)
+    rc, ac, iv, emin, rmin2 = pair_table_from_rmin_emin(rmin, eabs, ivor)
+    E = pose_energy(torch.tensor(mat, dtype=torch.float32, device=DEV), rc, ac, iv, emin, rmin2, system, ligs[0])
+    (torch.tensor(weights, dtype=torch.float32, device=DEV) * E).sum().backward()
+    g_rmin, g_eabs = rmin.grad.cpu().numpy(), eabs.grad.cpu().numpy()
+    l = w.ligands[0]
+
+    def total(rm, ea):
+        ffp = O.ff_from_par(rm * np.sqrt(3) / 2, 256 / 27 * ea, w.ivor)
+        En, _, _ = O.nb_energrad_complete(mat, l.coor, l.types - 1, w.coor_rec, w.rec_types - 1, ffp, g.origin,
+                                          g.spacing, g.dim, g.plateaudis, plateau=True)
+        return float((weights * En).sum())
+
+    idx = np.argsort(-np.abs(g_rmin).ravel())[:8]
+    assert np.abs(g_rmin).max() > 0
+    for flat in idx:
+        a, b = np.unravel_index(flat, g_rmin.shape)
+        for arr, gr in ((w.rmin, g_rmin), (w.emin_abs, g_eabs)):
+            h = 1e-4 * max(1.0, abs(arr[a, b]))
+            p_, m_ = arr.copy(), arr.copy()
+            p_[a, b] += h
+            m_[a, b] -= h
+            if arr is w.rmin:
+                fd = (total(p_, w.emin_abs) - total(m_, w.emin_abs)) / (2 * h)
+            else:
+                fd = (total(w.rmin, p_) - total(w.rmin, m_)) / (2 * h)
+            assert abs(gr[a, b] - fd) <= 5e-4 * max(abs(fd), 1e-2 * np.abs(gr).max()), (a, b, gr[a, b], fd)
