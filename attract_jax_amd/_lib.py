@@ -19,6 +19,7 @@ ATJ_TERM_NB = 0x2
 ATJ_NB_PLATEAU = 0x4
 ATJ_GRAD_EXACT = 0x8
 ATJ_POSE_QT = 0x10
+ATJ_GRAD_QT = 0x20
 
 ATJ_NSTATS = 4
 

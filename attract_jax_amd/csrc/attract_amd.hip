@@ -867,6 +867,35 @@ __device__ __forceinline__ void write_energrad(const KArgs& a, int p, float u, i
   else o[4 * (i - 13) + 3] = 0.f;  // i = 13..15: the zero column of rows 0..2
 }
 
+// Lane whose wave_sum_all result is accumulator i (the inverse of acc_index).
+__host__ __device__ constexpr int acc_lane(int i) {
+  return ((i & 1) << 5) | (((i >> 1) & 1) << 4) | (((i >> 2) & 1) << 3) | (((i >> 3) & 1) << 2);
+}
+
+// Energy and dE/d(q, t) of quaternion pose p (ATJ_GRAD_QT): the sums are gathered from
+// their lanes, the chain rule through as_matrix applied as in the optimizer update, and
+// lanes 0-15 write the 16 output words.
+__device__ void write_energrad_qt(const KArgs& a, int p, float u, int lane) {
+  float A[9], t[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) A[i] = lane_value(u, acc_lane(1 + i));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = lane_value(u, acc_lane(10 + i));
+  const float E = lane_value(u, acc_lane(0));
+  const float* pp = a.poses + (size_t)p * 7;
+  const float q[4] = {uni(pp[0]), uni(pp[1]), uni(pp[2]), uni(pp[3])};
+  float g[4];
+  quat_grad(q, A, g);
+  if (lane == 0) a.energy[p] = E;
+  if (!a.dEdM || lane >= 16) return;
+  float v = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v = lane == i ? g[i] : v;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v = lane == 4 + i ? t[i] : v;
+  a.dEdM[(size_t)p * 16 + lane] = v;
+}
+
 // One optimizer step of a pose from its evaluation (acc: energy, dE/dM) with the same
 // operations in the same order in the fused minimizer and in optimizer_step_kernel:
 // quaternion chain rule through as_matrix, optional gravity restraint w |t|^2
@@ -1245,8 +1274,13 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
         pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, w_pgrad, ws, P[0], P[1], st,
                                                              lane, u[0], u[1]);
         if (MODE != kAdjoint) {
-          write_energrad(a, p[0], u[0], lane);
-          if (p[1] >= 0) write_energrad(a, p[1], u[1], lane);
+          if (a.flags & ATJ_GRAD_QT) {
+            write_energrad_qt(a, p[0], u[0], lane);
+            if (p[1] >= 0) write_energrad_qt(a, p[1], u[1], lane);
+          } else {
+            write_energrad(a, p[0], u[0], lane);
+            if (p[1] >= 0) write_energrad(a, p[1], u[1], lane);
+          }
         }
         if (p[1] >= 0) ++b;
       }
@@ -1846,6 +1880,8 @@ int atj_energrad(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor
   if (N < 0 || njobs < 1) return fail(ATJ_ERR_INVALID, "N < 0 or no job");
   if (N == 0) return ATJ_OK;
   if (!poses || !energy) return fail(ATJ_ERR_INVALID, "poses / energy NULL");
+  if ((flags & ATJ_GRAD_QT) && !(flags & ATJ_POSE_QT))
+    return fail(ATJ_ERR_INVALID, "ATJ_GRAD_QT needs quaternion poses (ATJ_POSE_QT)");
   a.jobs = jobs_dev;
   a.pose_job = pose_job;
   a.pose_order = pose_order;

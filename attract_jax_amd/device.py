@@ -157,8 +157,6 @@ def lists_from_csr(counts, starts0, flat):
     counts = np.maximum(counts, 0)
     starts0 = np.asarray(starts0, np.int64).reshape(-1)
     flat = np.asarray(flat)
-    if len(flat) and int(flat.max()) >= 0xFFFF:
-        raise ValueError("neighbour lists need receptor indices < 65535")
     slot = np.where(counts > 0, (counts + LINE - 1) // LINE * LINE, 0)
     new_starts = LINE + np.concatenate(([0], np.cumsum(slot)[:-1]))
     total = LINE + int(slot.sum())
@@ -166,7 +164,12 @@ def lists_from_csr(counts, starts0, flat):
     n = int(counts.sum())
     if n:
         within = np.arange(n) - np.repeat(np.cumsum(counts) - counts, counts)
-        nblist[np.repeat(new_starts, counts) + within] = flat[np.repeat(starts0, counts) + within].astype(np.uint16)
+        entries = flat[np.repeat(starts0, counts) + within]
+        # only the entries the counts cover (a dense table's padding and pad_grid's sentinels
+        # lie beyond them)
+        if int(entries.max()) >= 0xFFFF:
+            raise ValueError("neighbour lists need receptor indices < 65535")
+        nblist[np.repeat(new_starts, counts) + within] = entries.astype(np.uint16)
     nbgrid = np.stack([counts, new_starts], axis=1).astype(np.int32)
     return nbgrid, nblist
 

@@ -11,6 +11,7 @@ ivor, emin, rmin2 (compacted, test-grid-energrad.py:67-79) or a ForceField.  Res
 are float32 torch tensors on the GPU.
 """
 import ctypes
+import hashlib
 
 import numpy as np
 import torch
@@ -176,24 +177,30 @@ class ScoringSystem:
         return torch.from_numpy(raw.copy()).to(self.device)
 
     def energrad(self, poses, ligand, *, pose_job=None, jobs=None, njobs=1, order=None, grad=True, flags=None,
-                 stats=False, qt=False, stream=None):
-        """energies (N,), dE/dM (N,4,4) of poses (N,4,4) [or (N,7) with qt=True]."""
+                 stats=False, qt=False, grad_qt=False, stream=None):
+        """energies (N,), dE/dM (N,4,4) of poses (N,4,4) [or (N,7) with qt=True; with grad_qt
+        as well the gradient is dE/d(q, t) (N,7), computed by the kernel]."""
         lib = _lib.load()
         d = self.device
         P = _f32(poses, d)
         N = P.shape[0]
         P = P.reshape(N, 7 if qt else 16)
+        if grad_qt and not qt:
+            raise ValueError("grad_qt needs quaternion poses (qt=True)")
         if jobs is None:
             jobs = self.jobs_tensor([ligand])
             njobs = 1
         energy = torch.empty(N, dtype=torch.float32, device=d)
         dEdM = torch.empty((N, 4, 4), dtype=torch.float32, device=d) if grad else None
-        fl = (self.flags if flags is None else flags) | (_lib.ATJ_POSE_QT if qt else 0)
+        fl = ((self.flags if flags is None else flags) | (_lib.ATJ_POSE_QT if qt else 0)
+              | (_lib.ATJ_GRAD_QT if grad_qt else 0))
         st = stream if stream is not None else _stream(d)
         _lib.check(lib.atj_energrad(ctypes.byref(self._gs), ctypes.byref(self._fs), ctypes.byref(self._rs),
                                     _ptr(jobs), njobs, _ptr(pose_job), _ptr(order), _ptr(P), N, _ptr(energy),
                                     _ptr(dEdM), fl, ctypes.byref(self.launch), _ptr(self.counter),
                                     _ptr(self.stats) if stats else ctypes.c_void_p(0), st))
+        if grad and grad_qt:
+            return energy, dEdM.reshape(N, 16)[:, :7]
         return energy, dEdM
 
     def bead_terms(self, poses, ligand, flags=None, qt=False):
@@ -215,11 +222,37 @@ class ScoringSystem:
         return grid4, voxel, nbe, nbg
 
 
+_system_cache = {}
+
+
+def _content_key(a):
+    """Key of a small host array (or tensor, or namedtuple of them) by its values."""
+    if isinstance(a, tuple):
+        return tuple(_content_key(x) for x in a)
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    a = np.ascontiguousarray(np.asarray(a))
+    return (a.dtype.str, a.shape, hashlib.blake2b(a.tobytes(), digest_size=16).digest())
+
+
 def _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
+    """ScoringSystem and Ligand of the reference-named calls.  They are made once per grid
+    object and per receptor / ligand / pair-table values: repeated calls (the reference's
+    jitted main called again on new poses) reuse the device copies."""
     dg = device_grid(grid)
+    ffk = _content_key(tuple(getattr(ff, f) for f in ("rc", "ac", "ivor", "emin", "rmin2")))
+    key = (id(dg), _content_key(coor_rec), _content_key(rec_atomtypes), _content_key(coor_lig),
+           _content_key(lig_atomtypes), _content_key(lig_atomtype_pos), ffk)
+    hit = _system_cache.get(key)
+    if hit is not None and hit[0] is dg:
+        return hit[1], hit[2]
     rec = Receptor.create(coor_rec, rec_atomtypes, dg.device)
     lig = Ligand.create(coor_lig, lig_atomtype_pos, lig_atomtypes, dg.device)
-    return ScoringSystem(dg, device_ff(ff), rec), lig
+    sysm = ScoringSystem(dg, device_ff(ff), rec)
+    if len(_system_cache) > 8:
+        _system_cache.clear()
+    _system_cache[key] = (dg, sysm, lig)
+    return sysm, lig
 
 
 def main(mat, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
@@ -237,22 +270,6 @@ def value_and_grad_main(mat, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, l
     return (e.sum(), e), g
 
 
-def quat_grad_from_dEdM(quat, dEdM):
-    """dE/dquat (N,4) through Rotation.as_matrix from dE/dM (torch, float32)."""
-    q = quat
-    A = dEdM[:, :3, :3]
-    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
-    g0 = 2 * (x * (A[:, 0, 0] - A[:, 1, 1] - A[:, 2, 2]) + y * (A[:, 0, 1] + A[:, 1, 0])
-              + z * (A[:, 0, 2] + A[:, 2, 0]) + w * (A[:, 2, 1] - A[:, 1, 2]))
-    g1 = 2 * (y * (A[:, 1, 1] - A[:, 0, 0] - A[:, 2, 2]) + x * (A[:, 0, 1] + A[:, 1, 0])
-              + z * (A[:, 1, 2] + A[:, 2, 1]) + w * (A[:, 0, 2] - A[:, 2, 0]))
-    g2 = 2 * (z * (A[:, 2, 2] - A[:, 0, 0] - A[:, 1, 1]) + x * (A[:, 0, 2] + A[:, 2, 0])
-              + y * (A[:, 1, 2] + A[:, 2, 1]) + w * (A[:, 1, 0] - A[:, 0, 1]))
-    g3 = 2 * (w * (A[:, 0, 0] + A[:, 1, 1] + A[:, 2, 2]) + x * (A[:, 2, 1] - A[:, 1, 2])
-              + y * (A[:, 0, 2] - A[:, 2, 0]) + z * (A[:, 1, 0] - A[:, 0, 1]))
-    return torch.stack([g0, g1, g2, g3], dim=1)
-
-
 def main_quat(quat, trans, coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid):
     """test4.py:396-401 with a single (non-conformer) ligand: (total, energies)."""
     sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
@@ -266,7 +283,6 @@ def value_and_grad_quat(quat, trans, coor_rec, rec_atomtypes, coor_lig, lig_atom
     """((total, energies), (dE/dquat (N,4), dE/dtrans (N,3))) -- test4.py:415."""
     sysm, lig = _system(coor_rec, rec_atomtypes, coor_lig, lig_atomtypes, lig_atomtype_pos, ff, grid)
     d = sysm.device
-    q = _f32(quat, d)
-    qt = torch.cat([q, _f32(trans, d)], dim=1)
-    e, g = sysm.energrad(qt, lig, grad=True, qt=True)
-    return (e.sum(), e), (quat_grad_from_dEdM(q, g), g[:, 3, :3].contiguous())
+    qt = torch.cat([_f32(quat, d), _f32(trans, d)], dim=1)
+    e, g = sysm.energrad(qt, lig, grad=True, qt=True, grad_qt=True)
+    return (e.sum(), e), (g[:, :4].contiguous(), g[:, 4:7].contiguous())

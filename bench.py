@@ -328,7 +328,9 @@ def main():
 
     if not args.no_cpu:
         from oracle.twin import Twin
-        ncores = min(16, os.cpu_count() or 1)
+        # every core this process may run on (the GPU box allots 16 of its host cores to a
+        # one-GPU job); os.cpu_count() is reported beside it
+        ncores = len(os.sched_getaffinity(0))
         grid = system.grid.to_reference_grid()
         ffa = [t.cpu().numpy() for t in (system.ff.rc, system.ff.ac, system.ff.ivor, system.ff.emin,
                                          system.ff.rmin2)]
@@ -361,8 +363,10 @@ def main():
             pass
         result["cpu_baseline"] = {
             "value": cps / tc, "unit": "pose-steps/s", "cores": ncores, "kind": "port",
-            "sample": f"FP32 C/OpenMP twin of the same algorithm: {per_lig} poses of each of the {len(twins)} "
-                      f"ligands x {w.nsteps} ADAM steps ({tc:.1f}s), {cpu_model}"}
+            "host_cpu_count": os.cpu_count(), "cpu_model": cpu_model,
+            "sample": f"FP32 C/OpenMP twin of the same algorithm on all {ncores} cores this process may use "
+                      f"(os.cpu_count() {os.cpu_count()}): {per_lig} poses of each of the {len(twins)} ligands x "
+                      f"{w.nsteps} ADAM steps ({tc:.1f}s), {cpu_model}"}
         log(f"[bench] cpu twin: {cps / tc:.3e} pose-steps/s on {ncores} threads over {tc:.1f}s")
 
     if not args.no_scoring:

@@ -46,6 +46,10 @@ extern "C" {
 #define ATJ_GRAD_EXACT 0x8u   /* gradient = exact derivative of the energy (finite-difference
                                  checkable) instead of the ATTRACT convention */
 #define ATJ_POSE_QT 0x10u     /* poses are 7 floats (qx, qy, qz, qw, tx, ty, tz), not 16 */
+#define ATJ_GRAD_QT 0x20u     /* atj_energrad with ATJ_POSE_QT: dEdM[p] receives the gradient
+                                 w.r.t. the pose's 7 parameters (dE/dq through as_matrix,
+                                 dE/dt), then 9 zeros (test4.py:415 differentiates the
+                                 quaternion form) */
 
 /* Device receptor grid.  Geometry follows read_grid.py:35-48. */
 typedef struct AtjGrid {
@@ -169,8 +173,9 @@ const char* atj_version(void);
  * poses: [N][16] (row-major 4x4) or [N][7] with ATJ_POSE_QT.
  * jobs/njobs give the ligand of each pose range; pose_order (nullable) is the
  * processing order of pose slots (a permutation of [0, N)).
- * energy: [N]; dEdM: [N][16] = dE/dM (nullable); counter: device int32 work
- * counter (caller-provided workspace, reset by the call); stats: nullable. */
+ * energy: [N]; dEdM: [N][16] = dE/dM (nullable; with ATJ_GRAD_QT dE/d(q, t), see the flag);
+ * counter: device int32 work counter (caller-provided workspace, reset by the call); stats:
+ * nullable. */
 int atj_energrad(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
                  const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
                  const int32_t* pose_order, const float* poses, int32_t N, float* energy,

@@ -80,3 +80,46 @@ class TinySystem:
     def oracle_energrad(self, mat=None, **kw):
         return O.energrad(self.mat if mat is None else mat, self.coor_lig, self.lig_types, self.lig_types,
                           self.coor_rec, self.rec_types, self.ff, self.grid, **kw)
+
+
+def pose_rmsd(Pa, Pb, coor_lig):
+    """Ligand RMSD between two sets of (quaternion, translation) poses (N, 7)."""
+    xa = O.transform(O.mat_from_qt(np.asarray(Pa, np.float64)[:, :4], np.asarray(Pa, np.float64)[:, 4:]), coor_lig)
+    xb = O.transform(O.mat_from_qt(np.asarray(Pb, np.float64)[:, :4], np.asarray(Pb, np.float64)[:, 4:]), coor_lig)
+    return O.rmsd(xa, xb)
+
+
+def twin_conditioning(twin, pose7, nsteps, lr_rot, lr_trans, coor_lig, ulps=4, tol=0.005, **kw):
+    """Minimize with the FP32 twin from the given starts and from starts whose translations are
+    `ulps` units in the last place larger.  A pose whose two results differ by more than `tol`
+    A RMSD amplifies rounding differences (the optimizer's normalised steps flip with the sign of
+    gradient components near zero): another FP32 implementation, whose sums round differently,
+    cannot be expected to follow it.  Returns (poses, energies, well_conditioned, rmsd)."""
+    P0 = np.array(pose7, np.float32).reshape(-1, 7)
+    Pa, Ea = twin.minimize(P0, nsteps, lr_rot, lr_trans, **kw)
+    P1 = P0.copy()
+    for _ in range(ulps):
+        P1[:, 4:] = np.nextafter(P1[:, 4:], np.float32(np.inf))
+    Pb, _ = twin.minimize(P1, nsteps, lr_rot, lr_trans, **kw)
+    r = pose_rmsd(Pa, Pb, coor_lig)
+    return Pa, Ea, r < tol, r
+
+
+def smooth_poses(mat, coor_lig, coor_rec, grid, margin):
+    """Poses whose beads all lie more than `margin` A from every voxel face of the grid they are
+    looked up in, from the inner grid's boundary (and the outer grid's, for beads outside the
+    inner one), and (for every receptor bead) from the plateau sphere."""
+    xp = O.transform(mat, coor_lig)
+    v, inside = O.voxelize(xp.reshape(-1, 3), grid.origin, grid.gridspacing, grid.dim)
+    vo = (v + grid.gridextension) / 2
+    frac = np.where(inside[:, None], v, vo)
+    cell = np.where(inside[:, None], grid.gridspacing, 2 * grid.gridspacing)
+    face = (np.abs(frac - np.round(frac)) * cell).min(axis=1)
+    edge = np.minimum(np.abs(v), np.abs(v - (np.asarray(grid.dim) - 1))).min(axis=1) * grid.gridspacing
+    if getattr(grid, "dim2", None) is not None:
+        edge2 = np.minimum(np.abs(vo), np.abs(vo - (np.asarray(grid.dim2) - 1))).min(axis=1) * 2 * grid.gridspacing
+        edge = np.minimum(edge, np.where(inside, np.inf, edge2))
+    d = np.linalg.norm(xp[:, :, None, :] - np.asarray(coor_rec)[None, None], axis=-1)
+    shell = np.abs(d - grid.plateaudis).min(axis=2).reshape(-1)
+    ok = (face > margin) & (edge > margin) & (shell > margin)
+    return ok.reshape(len(mat), -1).all(axis=1)

@@ -9,6 +9,7 @@ from attract_jax_amd.autodiff import dof_gradient, pair_table_from_rmin_emin, po
 from attract_jax_amd.pose import mat_from_qt, perturb_pose
 from attract_jax_amd.workloads import build_system, c1
 from oracle import oracle as O
+from tests.systems import smooth_poses
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -27,23 +28,6 @@ def _oracle_total(w, grid, ff, mat, weights):
     l = w.ligands[0]
     E, _ = O.energrad(mat, l.coor, l.types - 1, l.types - 1, w.coor_rec, w.rec_types - 1, ff, grid, exact=True)
     return float((weights * E).sum()), E
-
-
-def _smooth_poses(mat, coor_lig, coor_rec, grid, margin):
-    """Poses whose beads all lie more than `margin` A from every voxel face of the grid they are
-    looked up in, from the inner grid's boundary, and (for every receptor bead) from the
-    plateau sphere."""
-    xp = O.transform(mat, coor_lig)
-    v, inside = O.voxelize(xp.reshape(-1, 3), grid.origin, grid.gridspacing, grid.dim)
-    vo = (v + grid.gridextension) / 2
-    frac = np.where(inside[:, None], v, vo)
-    cell = np.where(inside[:, None], grid.gridspacing, 2 * grid.gridspacing)
-    face = (np.abs(frac - np.round(frac)) * cell).min(axis=1)
-    edge = np.minimum(np.abs(v), np.abs(v - (np.asarray(grid.dim) - 1))).min(axis=1) * grid.gridspacing
-    d = np.linalg.norm(xp[:, :, None, :] - np.asarray(coor_rec)[None, None], axis=-1)
-    shell = np.abs(d - grid.plateaudis).min(axis=2).reshape(-1)
-    ok = (face > margin) & (edge > margin) & (shell > margin)
-    return ok.reshape(len(mat), -1).all(axis=1)
 
 
 def test_param_gradient_vs_fd(sysc1):
@@ -110,7 +94,7 @@ def test_pose_dof_gradient_vs_fd(sysc1):
     # the energy is smooth in the pose only between voxel faces, the inner/outer boundary and
     # the plateau sphere: poses whose beads keep 2e-3 A from all of them (the perturbations
     # move a bead by at most 1e-4 x its distance from the ligand centre) must all agree
-    smooth = _smooth_poses(mat, l.coor, w.coor_rec, grid, margin=2e-3)
+    smooth = smooth_poses(mat, l.coor, w.coor_rec, grid, margin=2e-3)
     assert smooth.sum() >= 32, smooth.sum()
     assert err[smooth].max() < 2e-3, err[smooth].max()
     assert (err < 2e-3).mean() >= 0.9

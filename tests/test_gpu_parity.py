@@ -145,8 +145,15 @@ def test_exact_gradient_vs_fp64_fd(s1):
             mp[:, k, l] += h
             mm[:, k, l] -= h
             fd[:, k, l] = (s1.oracle(mp, exact=True)[0] - s1.oracle(mm, exact=True)[0]) / (2 * h)
-    ok = np.all(_grel(G, fd) < 1e-3, axis=(1, 2))
-    assert ok.mean() >= 0.9
+    err = _grel(G, fd).max(axis=(1, 2))
+    # every pose whose beads keep 2e-3 A from voxel faces, the inner-grid edge and the plateau
+    # sphere (where the energy is smooth over the perturbation) agrees within 1e-4
+    from tests.systems import smooth_poses
+    smooth = smooth_poses(mat.astype(np.float64), s1.w.ligands[0].coor, s1.w.coor_rec, s1.grid, margin=2e-3)
+    print(f"C1 exact dE/dM vs FP64 FD: {smooth.sum()} of {len(smooth)} poses smooth, max rel {err[smooth].max():.2e}")
+    assert smooth.sum() >= 32
+    assert err[smooth].max() < 1e-4
+    assert (err < 1e-3).mean() >= 0.9
 
 
 def test_reference_named_api(s1):
@@ -176,25 +183,51 @@ def test_reference_named_api(s1):
     Eo, Go = s1.oracle(M)
     assert _rel(Eq.cpu().numpy(), Eo).max() < 1e-4
     gq_o = O.quat_grad(q.astype(np.float64), Go[:, :3, :3])
-    assert (np.abs(gq.cpu().numpy() - gq_o) / np.maximum(np.abs(gq_o).max(1, keepdims=True), 1)).max() < 1e-3
+    sc = np.maximum(np.abs(np.concatenate([gq_o, Go[:, 3, :3]], 1)).max(1, keepdims=True), 1)
+    assert (np.abs(gq.cpu().numpy() - gq_o) / sc).max() < 1e-4
+    assert (np.abs(gt.cpu().numpy() - Go[:, 3, :3]) / sc).max() < 1e-4
+    # the device system is made once per argument values: the same call again gives the same
+    # bits, and other receptor coordinates give other energies
+    (_, Eq2), _ = A.value_and_grad_quat(q, w.trans[:200], w.coor_rec, w.rec_types - 1, lig.coor, lig.types - 1,
+                                        lig.types - 1, s1.ff, g)
+    assert torch.equal(Eq, Eq2)
+    (_, Eq3), _ = A.value_and_grad_quat(q, w.trans[:200], w.coor_rec + 0.5, w.rec_types - 1, lig.coor,
+                                        lig.types - 1, lig.types - 1, s1.ff, g)
+    assert not torch.equal(Eq, Eq3)
 
 
 def test_minimize_vs_twin_and_oracle(s1):
+    """C1, 1000 poses x 100 ADAM steps.  Every pose whose FP32 trajectory is well conditioned (the
+    twin started 4 ulps away ends within 0.005 A) ends within 0.05 A RMSD of the twin, with an
+    energy within 1e-4 relative wherever the poses agree to 1e-4 A, and the 200 of them minimized by
+    the FP64 oracle within 0.05 A of it; the ill-conditioned poses are named by that test (their
+    count is bounded)."""
+    from tests.systems import pose_rmsd, twin_conditioning
     w = s1.w
     plan = BatchPlan(s1.system, s1.ligs, [w.npose], w.nsteps, w.lr_rot, w.lr_trans)
     st = PoseState.create(w.quat, w.trans, DEV)
     minimize(s1.system, plan, st, w.nsteps)
     P = st.pose.cpu().numpy()
     assert np.all(st.count.cpu().numpy() == w.nsteps)
-    Pt, Et = s1.twins[0].minimize(np.concatenate([w.quat, w.trans], 1), w.nsteps, w.lr_rot, w.lr_trans)
     lig = w.ligands[0]
-    xg = O.transform(O.mat_from_qt(P[:, :4].astype(np.float64), P[:, 4:].astype(np.float64)), lig.coor)
-    xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
-    r = O.rmsd(xg, xt)
-    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
+    Pt, Et, good, rc = twin_conditioning(s1.twins[0], np.concatenate([w.quat, w.trans], 1), w.nsteps, w.lr_rot,
+                                         w.lr_trans, lig.coor)
+    r = pose_rmsd(P, Pt, lig.coor)
     Eg = st.energy.cpu().numpy()
-    close = r < 1e-3
-    assert _rel(Eg[close], Et[close]).max() < 1e-3
+    print(f"C1 minimization vs twin: {good.sum()} of {len(good)} poses well conditioned; RMSD max "
+          f"{r[good].max():.2e} median {np.median(r[good]):.2e} A; energy rel max {_rel(Eg[good], Et[good]).max():.2e}"
+          f"; all poses: RMSD median {np.median(r):.2e}, 99th pct {np.percentile(r, 99):.2e}")
+    assert good.mean() >= 0.9
+    assert r[good].max() < 0.05
+    # the energy follows the pose: where the poses agree to 1e-4 A, 99% of the energies agree to
+    # 1e-4 and all to 1e-3 (a pose still moving feels 100 kcal/mol/A: 1e-4 A is then 1e-2)
+    tight = good & (r < 1e-4)
+    er = _rel(Eg, Et)
+    print(f"  energy rel where RMSD < 1e-4 A: median {np.median(er[tight]):.1e}, 99th pct "
+          f"{np.percentile(er[tight], 99):.1e}, max {er[tight].max():.1e}")
+    assert tight.mean() >= 0.9
+    assert np.percentile(er[tight], 99) < 1e-4 and er[good].max() < 1e-3
+    assert np.median(r) < 0.01
 
     sub = slice(0, 200)
 
@@ -202,9 +235,12 @@ def test_minimize_vs_twin_and_oracle(s1):
         return s1.oracle(O.mat_from_qt(q, t))
 
     qo, to, Eo = O.adam_minimize(w.quat[sub], w.trans[sub], w.nsteps, w.lr_rot, w.lr_trans, fn)
-    xo = O.transform(O.mat_from_qt(qo, to), lig.coor)
-    r64 = O.rmsd(xg[sub], xo)
-    assert np.median(r64) < 0.05 and (r64 < 0.05).mean() >= 0.8
+    r64 = pose_rmsd(P[sub], np.concatenate([qo, to], 1), lig.coor)
+    g = good[sub]
+    print(f"C1 minimization vs FP64 oracle (200 poses): well conditioned RMSD max {r64[g].max():.2e} median "
+          f"{np.median(r64[g]):.2e} A, energy rel max {_rel(Eg[sub][g], Eo[g]).max():.2e}")
+    assert r64[g].max() < 0.05
+    assert _rel(Eg[sub][g], Eo[g]).max() < 1e-3
 
 
 def test_batched_equals_plain(s1):

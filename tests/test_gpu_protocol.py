@@ -16,6 +16,7 @@ from attract_jax_amd.protocol import ConformerSet, dock_test5, prepare, run_sche
 from oracle import oracle as O
 from oracle.twin import Twin
 
+from .systems import pose_rmsd, twin_conditioning
 from .test_gpu_parity import Sys
 
 pytestmark = pytest.mark.gpu
@@ -52,14 +53,17 @@ def test_adabelief_gravity_vs_twin_and_plain(s1):
     b = PoseState.create(w.quat[:n], w.trans[:n], DEV)
     minimize_plain(s1.system, s1.ligs[0], b, nst, lr, lr, eps=1e-16, **ADAB)
     assert torch.equal(a.pose, b.pose) and torch.equal(a.energy, b.energy)
-    Pt, Et = s1.twins[0].minimize(np.concatenate([w.quat[:n], w.trans[:n]], 1), nst, lr, lr, eps=1e-16, kind=1,
-                                  eps_root=1e-16, gravity=0.1)
+    Pt, Et, good, _ = twin_conditioning(s1.twins[0], np.concatenate([w.quat[:n], w.trans[:n]], 1), nst, lr, lr,
+                                        w.ligands[0].coor, eps=1e-16, kind=1, eps_root=1e-16, gravity=0.1)
     r = _rmsd_to_twin(w.ligands[0].coor, a.pose.cpu().numpy(), Pt)
-    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.95
-    close = r < 1e-4
     E = a.energy.cpu().numpy()
-    assert close.mean() > 0.5
-    assert (np.abs(E[close] - Et[close]) / np.maximum(np.abs(Et[close]), 1.0)).max() < 1e-3
+    erel = np.abs(E - Et) / np.maximum(np.abs(Et), 1.0)
+    # every well-conditioned pose (the twin from starts 4 ulps away ends within 0.005 A)
+    assert good.mean() >= 0.9
+    tight = good & (r < 1e-4)
+    assert r[good].max() < 0.05 and tight.mean() >= 0.5
+    assert np.percentile(erel[tight], 99) < 1e-4 and erel[good].max() < 1e-3
+    assert np.median(r) < 0.01
     # the reported energy contains the restraint: far poses (no receptor terms) give 0.1 |t|^2
     far = np.concatenate([w.quat[:2], np.array([[300.0, 0, 0], [0, -400.0, 0]])], 1)
     c = PoseState.create(far[:, :4], far[:, 4:], DEV)
@@ -74,9 +78,21 @@ def test_schedule_vs_twin_stage_by_stage(s1):
     st = PoseState.create(w.quat[:n], w.trans[:n], DEV)
     run_schedule(s1.system, plan, st, SHORT)
     assert np.all(st.count.cpu().numpy() == SHORT[-1][0])
-    Pt, Et = _twin_schedule(s1.twins[0], np.concatenate([w.quat[:n], w.trans[:n]], 1), SHORT)
+    start = np.concatenate([w.quat[:n], w.trans[:n]], 1).astype(np.float32)
+    Pt, Et = _twin_schedule(s1.twins[0], start, SHORT)
+    moved = start.copy()
+    for _ in range(4):
+        moved[:, 4:] = np.nextafter(moved[:, 4:], np.float32(np.inf))
+    Pm, _ = _twin_schedule(s1.twins[0], moved, SHORT)
+    good = pose_rmsd(Pt, Pm, w.ligands[0].coor) < 0.005
     r = _rmsd_to_twin(w.ligands[0].coor, st.pose.cpu().numpy(), Pt)
-    assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.9
+    E = st.energy.cpu().numpy()
+    erel = np.abs(E - Et) / np.maximum(np.abs(Et), 1.0)
+    tight = good & (r < 1e-4)
+    assert good.mean() >= 0.9 and tight.mean() >= 0.5
+    assert r[good].max() < 0.05
+    assert np.percentile(erel[tight], 99) < 1e-4 and erel[good].max() < 1e-3
+    assert np.median(r) < 0.01
 
 
 def test_conformer_library_jobs(s1):

@@ -13,6 +13,7 @@ from attract_jax_amd.pose import mat_from_qt
 from attract_jax_amd.workloads import build_system, c3
 from oracle import oracle as O
 from oracle.twin import Twin
+from tests.systems import pose_rmsd, twin_conditioning
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -52,12 +53,22 @@ def test_full_batched_minimization_properties():
 
         lig = w.ligands[k]
         twin = Twin(grid, ffa, w.coor_rec, w.rec_types - 1, lig.coor, lig.types - 1, lig.types - 1)
-        idx = np.arange(int(off[k]), int(off[k]) + 4096, 512)
-        Pt, _ = twin.minimize(np.concatenate([w.quat[idx], w.trans[idx]], 1), w.nsteps, w.lr_rot, w.lr_trans)
-        xg = O.transform(O.mat_from_qt(P[idx, :4].astype(np.float64), P[idx, 4:].astype(np.float64)), lig.coor)
-        xt = O.transform(O.mat_from_qt(Pt[:, :4].astype(np.float64), Pt[:, 4:].astype(np.float64)), lig.coor)
-        r = O.rmsd(xg, xt)
-        assert np.median(r) < 0.01 and (r < 0.05).mean() >= 0.75, (k, r)
+        idx = np.arange(int(off[k]), int(off[k]) + 4096, 64)
+        Pt, Et, good, rc = twin_conditioning(twin, np.concatenate([w.quat[idx], w.trans[idx]], 1), w.nsteps,
+                                             w.lr_rot, w.lr_trans, lig.coor)
+        r = pose_rmsd(P[idx], Pt, lig.coor)
+        Eg = st.energy.cpu().numpy()[idx]
+        erel = np.abs(Eg - Et) / np.maximum(np.abs(Et), 1.0)
+        print(f"C3 ligand {k} ({lig.coor.shape[0]} beads): {good.sum()} of {len(idx)} poses well conditioned; "
+              f"RMSD vs twin max {r[good].max():.2e} median {np.median(r[good]):.2e} A, energy rel max "
+              f"{erel[good].max():.2e}; all: RMSD median {np.median(r):.2e}")
+        assert good.mean() >= 0.8, (k, good.mean())
+        assert r[good].max() < 0.05, (k, r[good].max())
+        tight = good & (r < 1e-4)
+        print(f"  energy rel where RMSD < 1e-4 A ({tight.sum()} poses): median {np.median(erel[tight]):.1e}, "
+              f"99th pct {np.percentile(erel[tight], 99):.1e}")
+        assert tight.mean() >= 0.5, (k, tight.mean())
+        assert np.percentile(erel[tight], 99) < 1e-4 and erel[good].max() < 1e-3, (k, erel[good].max())
 
 
 def test_full_single_shot_scoring_and_adjoint_properties():
