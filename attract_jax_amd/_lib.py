@@ -106,6 +106,12 @@ class AtjPoseState(ctypes.Structure):
 
 
 ATJ_LAUNCH_REC_GLOBAL = 0x1
+ATJ_LAUNCH_CLAIM_MASK = 0xF0
+
+
+def ATJ_LAUNCH_CLAIM(n):
+    """Launch option: n poses per work claim (1, 2, 4 or 8)."""
+    return int(n) << 4
 
 
 class AtjLaunch(ctypes.Structure):

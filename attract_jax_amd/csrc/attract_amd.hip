@@ -1755,7 +1755,7 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   const int resident = per_cu * d.cus;
   const bool auto_blocks = blocks == 0;
   if (auto_blocks) blocks = resident;
-  a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
+  if (a.batch == 0) a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
   if (auto_blocks) {  // no more blocks than the first round of claims needs
     const long long per_block = (long long)wpb * a.batch;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
@@ -1801,7 +1801,11 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   if (MODE == kAdjoint) wpb = min(wpb, kMaxAdjointWaves);
   const uint32_t options = launch ? launch->options : 0u;
-  if (options & ~ATJ_LAUNCH_REC_GLOBAL) return fail(ATJ_ERR_INVALID, "unknown launch option");
+  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_CLAIM_MASK)) return fail(ATJ_ERR_INVALID, "unknown launch option");
+  const int claim = (int)((options & ATJ_LAUNCH_CLAIM_MASK) >> 4);
+  if (claim != 0 && claim != 1 && claim != 2 && claim != 4 && claim != kBatch)
+    return fail(ATJ_ERR_INVALID, "poses per claim must be 1, 2, 4 or 8");
+  a.batch = claim;  // 0: chosen at launch
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t fixed = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2);
   const size_t per_wave = sizeof(WaveScratch) +

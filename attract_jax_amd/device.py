@@ -12,6 +12,7 @@ kernels of libattract_amd.so.  Layouts in HBM:
 * receptor beads: float4 (x, y, z, pair-table row as int32 bits);
 * ligand beads: float4 (x, y, z, grid type | pair-table column << 16 as int32 bits).
 """
+import copy
 import ctypes
 from dataclasses import dataclass, field
 
@@ -231,6 +232,23 @@ class DeviceGrid:
         _lib.check(lib.atj_grid_pack_corners(_ptr(plain), self.ntypes, ctypes.byref(d), _ptr(packed),
                                              _stream(self.device)))
         return packed
+
+    def with_layout(self, layout):
+        """This grid in another potential layout, sharing the neighbour lists and their shell
+        order (a view: the lists are not copied).  The plain potentials must be kept
+        (keep_plain=True) unless this grid is plain already."""
+        if layout == self.layout:
+            return self
+        if self.inner is not None and self.inner_plain is None:
+            raise ValueError("the plain potentials were not kept (keep_plain=True)")
+        v = copy.copy(self)
+        v.layout = layout
+        if layout == _lib.ATJ_LAYOUT_PLAIN:
+            v.inner, v.outer = self.inner_plain, self.outer_plain
+        else:
+            v.inner = v._relayout(self.inner_plain, self.dim)
+            v.outer = v._relayout(self.outer_plain, self.dim2) if self.outer_plain is not None else None
+        return v
 
     @property
     def has_potentials(self):

@@ -182,8 +182,9 @@ class Scheduler:
     def _admit(self, now):
         """Admit pending jobs in arrival order while they fit.  A job that would not fit even
         with the device to itself fails at once; one that fits later waits (and holds back
-        the jobs behind it).  A group's slot pool grows by what the job lacks at its end; an
-        idle group too small for the job is dropped and made again at the job's size."""
+        the jobs behind it).  A busy group's slot pool doubles, or grows by what the job lacks
+        at its end when doubling does not fit; an idle group too small for the job gets a new
+        pool of the job's size (its grid stays resident)."""
         while self.pending:
             job = self.pending[0]
             key = job.spec.receptor_key
@@ -197,18 +198,26 @@ class Scheduler:
             if g is not None:
                 start = g.pool.alloc(job.npose)
                 if start < 0 and not g.jobs and g.inflight is None:
-                    self._drop(g)
-                    g = None
+                    cap = max(self.initial_slots, job.npose)
+                    delta = self._group_bytes(cap) - self._group_bytes(g.pool.capacity)
+                    if not self._evict_for(delta, keep=g):
+                        return  # wait until running jobs free memory
+                    self.backend.reset_group(g, cap)
+                    g.pool = SlotPool(cap)
+                    self.used += delta
+                    start = g.pool.alloc(job.npose)
                 elif start < 0:
                     if g.inflight is not None:
                         return  # slot arrays are resized only between chunks
-                    extra = job.npose - g.pool.tail_free()
-                    need = self._group_bytes(extra)
-                    if not self._evict_for(need, keep=g):
-                        return  # wait until running jobs free memory
+                    lack = job.npose - g.pool.tail_free()
+                    extra = max(lack, g.pool.capacity)
+                    if not self._evict_for(self._group_bytes(extra), keep=g):
+                        extra = lack
+                        if not self._evict_for(self._group_bytes(extra), keep=g):
+                            return  # wait until running jobs free memory
                     self.backend.grow_group(g, g.pool.capacity + extra)
                     g.pool.grow(g.pool.capacity + extra)
-                    self.used += need
+                    self.used += self._group_bytes(extra)
                     start = g.pool.alloc(job.npose)
             if g is None:
                 cap = max(self.initial_slots, job.npose)
@@ -357,6 +366,18 @@ class GpuBackend:
         from .minimize import PoseState
         with torch.cuda.stream(g.handle["stream"]):
             g.handle["state"] = PoseState(grow(st.pose), grow(st.m), grow(st.v), grow(st.count), grow(st.energy))
+
+    def reset_group(self, g, capacity):
+        """Fresh slot arrays of `capacity` slots for an idle group (its grid stays)."""
+        from .minimize import PoseState
+        torch = self.torch
+        z = torch.zeros
+        with torch.cuda.stream(g.handle["stream"]):
+            g.handle["state"] = PoseState(z((capacity, 7), device=self.device), z((capacity, 7), device=self.device),
+                                          z((capacity, 7), device=self.device),
+                                          z(capacity, dtype=torch.int32, device=self.device),
+                                          z(capacity, device=self.device))
+        g.handle["plan"] = None
 
     def drop_group(self, g):
         self._built.pop(g.key, None)

@@ -156,12 +156,16 @@ typedef struct AtjPoseState {
 
 /* Launch configuration of the pose kernels (chosen by the autotuner). */
 #define ATJ_LAUNCH_REC_GLOBAL 0x1u  /* read receptor beads through the vector-memory (L1) path, not LDS */
+#define ATJ_LAUNCH_CLAIM(n) ((uint32_t)(n) << 4)  /* poses per claim, n = 1, 2, 4 or 8 (bits 4-7;
+                                                    0 = chosen from N and the wavefronts) */
+#define ATJ_LAUNCH_CLAIM_MASK 0xF0u
 typedef struct AtjLaunch {
   int32_t waves_per_block;  /* 1..16 (0 = default 4); the adjoint uses at most 12 */
   int32_t blocks;           /* 0 = as many as the device holds, fewer for small launches */
   uint32_t options;         /* ATJ_LAUNCH_* */
 } AtjLaunch;
-/* Poses are claimed 1, 2, 4 or 8 at a time (chosen from N and the wavefronts launched); the
+/* Poses are claimed 1, 2, 4 or 8 at a time (chosen from N and the wavefronts launched, or set
+ * by ATJ_LAUNCH_CLAIM); the
  * first claim of each wavefront in the blocks the device holds at once is fixed by its
  * index, further claims come from the work counter. */
 

@@ -31,6 +31,9 @@ class FakeBackend:
     def grow_group(self, g, cap):
         g.handle["grown"] = cap
 
+    def reset_group(self, g, cap):
+        g.handle["reset"] = cap
+
     def drop_group(self, g):
         self.dropped.append(g.key)
 
@@ -185,7 +188,8 @@ def test_long_job_is_not_starved():
 
 def test_idle_group_too_small_is_made_again_at_the_job_size():
     """Grid 100 B, 100 B per slot, 16 initial slots, budget 5000 B: after a 10-pose job the
-    idle 16-slot group cannot grow by 40 (5700 B) but a new 40-slot group fits (4100 B)."""
+    idle 16-slot group cannot grow by 40 (5700 B) but a fresh 40-slot pool fits (4100 B); the
+    grid stays resident."""
     clock = FakeClock()
     be = FakeBackend(clock, {0: 100})
     s = Scheduler(be, hbm_budget=5000, chunk_steps=10, initial_slots=16, clock=clock)
@@ -194,20 +198,36 @@ def test_idle_group_too_small_is_made_again_at_the_job_size():
     b = s.submit(spec(0, 40, 20))
     drive(s, clock, limit=1000)
     assert a.state == b.state == "done"
-    assert s.groups[0].pool.capacity == 40 and s.used <= 5000
+    assert s.groups[0].pool.capacity == 40 and s.used == 4100
+    assert be.dropped == [] and s.groups[0].handle["reset"] == 40
 
 
-def test_pool_grows_by_what_the_job_lacks():
+def test_pool_grows_by_what_the_job_lacks_when_doubling_does_not_fit():
+    clock = FakeClock()
+    be = FakeBackend(clock, {0: 100})
+    s = Scheduler(be, hbm_budget=3200, chunk_steps=10, initial_slots=16, clock=clock)
+    a = s.submit(spec(0, 10, 200))
+    s.tick()
+    b = s.submit(spec(0, 20, 20))  # 6 free at the end: doubling (32) would not fit, +14 does
+    for _ in range(5):
+        clock.t += 1.0
+        s.tick()
+    assert b.state in ("admitted", "done") and s.groups[0].pool.capacity == 30
+    drive(s, clock)
+    assert a.state == b.state == "done"
+
+
+def test_pool_doubles_when_it_fits():
     clock = FakeClock()
     be = FakeBackend(clock, {0: 100})
     s = Scheduler(be, hbm_budget=10 ** 6, chunk_steps=10, initial_slots=16, clock=clock)
     a = s.submit(spec(0, 10, 200))
     s.tick()
-    b = s.submit(spec(0, 20, 20))  # 6 slots free at the end: grows by 14
+    b = s.submit(spec(0, 20, 20))  # 6 slots free at the end: doubles to 32 slots
     for _ in range(5):
         clock.t += 1.0
         s.tick()
-    assert b.state in ("admitted", "done") and s.groups[0].pool.capacity == 30
+    assert b.state in ("admitted", "done") and s.groups[0].pool.capacity == 32
     drive(s, clock)
     assert a.state == b.state == "done"
 
