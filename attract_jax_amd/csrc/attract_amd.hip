@@ -1460,24 +1460,83 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int n, 
 // Grid construction
 // ---------------------------------------------------------------------------
 
-// Potential + force (ATTRACT sign, = -dE/dp) of ligand grid type blockIdx.y at
-// every grid point; plateau-capped inside plateaudis, zero beyond cutoff
-// (test-calc-grid-energy.py:132-148); the force of a capped receptor bead is the
-// plateau gradient projected on the plateau sphere (the complement of the
-// force-shift of test-grid-energrad.py:140-146).
-__global__ void build_potentials_kernel(GridView G, const float4* rec, int nrec, const float* rc, const float* ac,
-                                        const float* ivor, const float* emin, const float* rmin2, int nrt,
-                                        int nlt, const int* lig_fftype, float cutoff2, float* out, int dx, int dy,
-                                        int dz, int outer) {
-  const int t = blockIdx.y;
+// Potential + force (ATTRACT sign, = -dE/dp) of the ligand grid types at every grid point:
+// plateau-capped inside plateaudis, zero beyond cutoff (test-calc-grid-energy.py:132-148); the
+// force of a capped receptor bead is the plateau gradient projected on the plateau sphere (the
+// complement of the force-shift of test-grid-energrad.py:140-146).
+//
+// One thread per grid point computes up to kBuildTypes ligand types (blockIdx.y selects the
+// group).  Every distance it meets is at least plateaudis (capped), so a pair's energy is
+// ivor (rc u^4 - ac u^3) with u = 1/max(dsq, P^2), its branch below rmin2 only mattering for
+// pair types with ivor != 1 and rmin2 > P^2.  The potential of type t at the point therefore
+// factorises over receptor types rt:
+//   E_t = sum_rt ivor rc(rt, t) U4(rt) - ivor ac(rt, t) U3(rt),
+//   F_t = sum_rt 4 ivor rc(rt, t) V5(rt) - 3 ivor ac(rt, t) V4(rt),
+// with U4 = sum u^4, U3 = sum u^3, V5 = sum 2 c u^5 d, V4 = sum 2 c u^4 d over the beads of type
+// rt within the cutoff (d = point - bead, c = P / |d| for capped beads, else 1).  The bead loop
+// computes the 8 sums once for all types; the pair types whose lower branch can hold get their
+// difference added bead by bead.  Beads are staged in LDS in type order (a stable counting sort
+// per block, so sums run in the same order in every block and every run).
+constexpr int kBuildTypes = 32;
+constexpr int kBuildChunk = 2048;  // receptor beads staged per pass
+constexpr int kBuildThreads = 256;
+
+__device__ void stage_by_type(const float4* rec, int c0, int n, int nrt, float4* s_bead, int* s_off, int* s_cur,
+                              int* s_type) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i <= nrt; i += blockDim.x) s_off[i] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int t = (int)min((unsigned)__float_as_int(rec[c0 + i].w), (unsigned)(nrt - 1));
+    atomicAdd(&s_off[t + 1], 1);  // integer counts: the order of the increments does not matter
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int i = 1; i <= nrt; ++i) s_off[i] += s_off[i - 1];
+  __syncthreads();
+  for (int i = tid; i < nrt; i += blockDim.x) s_cur[i] = s_off[i];
+  __syncthreads();
+  // rounds of blockDim beads: a bead's place is its type's cursor plus the beads of its type
+  // earlier in the round (stable, so the bead order within a type is the receptor order)
+  for (int r0 = 0; r0 < n; r0 += blockDim.x) {
+    const int i = r0 + tid;
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    int t = -1;
+    if (i < n) {
+      b = rec[c0 + i];
+      t = (int)min((unsigned)__float_as_int(b.w), (unsigned)(nrt - 1));
+      b.w = __int_as_float(t);
+    }
+    s_type[tid] = t;
+    __syncthreads();
+    int rank = 0, later = 0;
+    if (t >= 0) {
+      for (int j = 0; j < tid; ++j) rank += s_type[j] == t;
+      for (int j = tid + 1; j < (int)blockDim.x; ++j) later += s_type[j] == t;
+      s_bead[s_cur[t] + rank] = b;
+    }
+    __syncthreads();
+    if (t >= 0 && later == 0) s_cur[t] += rank + 1;  // the round's last bead of the type
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBuildThreads)
+    build_potentials_kernel(GridView G, const float4* rec, int nrec, const float* rc, const float* ac,
+                            const float* ivor, const float* emin, const float* rmin2, int nrt, int nlt,
+                            const int* lig_fftype, int ntypes, float cutoff2, float* out, int dx, int dy, int dz,
+                            int outer) {
+  extern __shared__ float4 smem[];
+  float4* s_bead = smem;
+  int* s_off = reinterpret_cast<int*>(s_bead + kBuildChunk);
+  int* s_cur = s_off + (nrt + 1);
+  int* s_type = s_cur + nrt;
+  const int t0 = blockIdx.y * kBuildTypes;
+  const int ng = min(kBuildTypes, ntypes - t0);
   const long long nv = (long long)dx * dy * dz;
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  // types beyond the pair table are read as its last row / column (memory-safe for any input)
-  const int lt = (int)min((unsigned)lig_fftype[t], (unsigned)(nlt - 1));
-  const float P2 = G.p2;
-  const float P = sqrtf(P2);
-  if (v >= nv) return;
-  const int iz = (int)(v % dz), iy = (int)((v / dz) % dy), ix = (int)(v / ((long long)dy * dz));
+  const long long vc = v < nv ? v : nv - 1;  // threads past the end compute a copy and store nothing
+  const int iz = (int)(vc % dz), iy = (int)((vc / dz) % dy), ix = (int)(vc / ((long long)dy * dz));
   float px, py, pz;
   if (outer) {
     px = G.ox + G.spacing * (2.f * ix - G.ge);
@@ -1488,34 +1547,85 @@ __global__ void build_potentials_kernel(GridView G, const float4* rec, int nrec,
     py = G.oy + G.spacing * iy;
     pz = G.oz + G.spacing * iz;
   }
-  float E = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
-  for (int r = 0; r < nrec; ++r) {
-    const float4 rb = rec[r];
-    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
-    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-    if (!(dsq < cutoff2)) continue;
-    const int pi = __float_as_int(clamp_rec_type(rb, nrt).w) * nlt + lt;
-    const float c_rc = rc[pi], c_ac = ac[pi], iv = ivor[pi], em = emin[pi], r2 = rmin2[pi];
-    const float dd = dsq < P2 ? P2 : dsq;
-    const float u = 1.f / dd;
-    const float u3 = (u * u) * u;
-    const float rep = c_rc * u;
-    const float vlj = (rep - c_ac) * u3;
-    const float dv = -(fmaf(4.f, rep, -3.f * c_ac) * u3) * u;
-    const bool lo = dd < r2;
-    const float e = lo ? fmaf(iv - 1.f, em, vlj) : iv * vlj;
-    const float dg = lo ? dv : iv * dv;
-    E += e;
-    float gs;
-    if (dsq < P2)
-      gs = dsq > 0.f ? 2.f * dg * P / sqrtf(dsq) : 0.f;
-    else
-      gs = 2.f * dg;
-    fx = fmaf(-gs, ddx, fx);
-    fy = fmaf(-gs, ddy, fy);
-    fz = fmaf(-gs, ddz, fz);
+  const float P2 = G.p2;
+  const float P = sqrtf(P2);
+  float E[kBuildTypes], Fx[kBuildTypes], Fy[kBuildTypes], Fz[kBuildTypes];
+#pragma unroll
+  for (int k = 0; k < kBuildTypes; ++k) E[k] = Fx[k] = Fy[k] = Fz[k] = 0.f;
+  for (int c0 = 0; c0 < nrec; c0 += kBuildChunk) {
+    const int n = min(kBuildChunk, nrec - c0);
+    __syncthreads();  // the previous chunk's readers are done
+    stage_by_type(rec, c0, n, nrt, s_bead, s_off, s_cur, s_type);
+    for (int rt = 0; rt < nrt; ++rt) {
+      const int b0 = s_off[rt], b1 = s_off[rt + 1];
+      if (b0 == b1) continue;
+      float U4 = 0.f, U3 = 0.f, V5[3] = {0.f, 0.f, 0.f}, V4[3] = {0.f, 0.f, 0.f};
+      for (int b = b0; b < b1; ++b) {
+        const float4 rb = s_bead[b];
+        const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+        const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+        const bool in = dsq < cutoff2;
+        const bool capped = dsq < P2;
+        const float dd = capped ? P2 : dsq;
+        const float u = 1.f / dd;
+        const float u3 = (u * u) * u;
+        const float u4 = u3 * u;
+        const float c = capped ? (dsq > 0.f ? P / sqrtf(dsq) : 0.f) : 1.f;
+        const float w4 = in ? 2.f * c * u4 : 0.f;
+        const float w5 = w4 * u;
+        U4 += in ? u4 : 0.f;
+        U3 += in ? u3 : 0.f;
+        V5[0] = fmaf(w5, ddx, V5[0]);
+        V5[1] = fmaf(w5, ddy, V5[1]);
+        V5[2] = fmaf(w5, ddz, V5[2]);
+        V4[0] = fmaf(w4, ddx, V4[0]);
+        V4[1] = fmaf(w4, ddy, V4[1]);
+        V4[2] = fmaf(w4, ddz, V4[2]);
+      }
+#pragma unroll
+      for (int k = 0; k < kBuildTypes; ++k) {
+        if (k >= ng) continue;
+        // types beyond the pair table are read as its last row / column (memory-safe for any input)
+        const int lt = (int)min((unsigned)lig_fftype[t0 + k], (unsigned)(nlt - 1));
+        const int pi = rt * nlt + lt;
+        const float iv = ivor[pi];
+        const float a = iv * rc[pi], bb = iv * ac[pi];
+        E[k] = fmaf(a, U4, fmaf(-bb, U3, E[k]));
+        Fx[k] = fmaf(4.f * a, V5[0], fmaf(-3.f * bb, V4[0], Fx[k]));
+        Fy[k] = fmaf(4.f * a, V5[1], fmaf(-3.f * bb, V4[1], Fy[k]));
+        Fz[k] = fmaf(4.f * a, V5[2], fmaf(-3.f * bb, V4[2], Fz[k]));
+        const float r2 = rmin2[pi];
+        if (iv != 1.f && r2 > P2) {
+          // the lower branch below rmin2: energy (1 - ivor)(vlj - emin), force from (1 - ivor) dvlj
+          const float em = emin[pi], crc = rc[pi], cac = ac[pi];
+          for (int b = b0; b < b1; ++b) {
+            const float4 rb = s_bead[b];
+            const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+            const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+            const bool capped = dsq < P2;
+            const float dd = capped ? P2 : dsq;
+            if (!(dsq < cutoff2) || !(dd < r2)) continue;
+            const float u = 1.f / dd;
+            const float u3 = (u * u) * u;
+            const float vlj = (crc * u - cac) * u3;
+            const float dv = -(fmaf(4.f, crc * u, -3.f * cac) * u3) * u;
+            const float c = capped ? (dsq > 0.f ? P / sqrtf(dsq) : 0.f) : 1.f;
+            const float gs = 2.f * (1.f - iv) * dv * c;
+            E[k] += (1.f - iv) * (vlj - em);
+            Fx[k] = fmaf(-gs, ddx, Fx[k]);
+            Fy[k] = fmaf(-gs, ddy, Fy[k]);
+            Fz[k] = fmaf(-gs, ddz, Fz[k]);
+          }
+        }
+      }
+    }
   }
-  *reinterpret_cast<float4*>(out + ((size_t)t * nv + v) * 4) = make_float4(E, fx, fy, fz);
+  if (v >= nv) return;
+#pragma unroll
+  for (int k = 0; k < kBuildTypes; ++k) {
+    if (k >= ng) continue;
+    *reinterpret_cast<float4*>(out + ((size_t)(t0 + k) * nv + v) * 4) = make_float4(E[k], Fx[k], Fy[k], Fz[k]);
+  }
 }
 
 __global__ void count_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis2, int* counts) {
@@ -2054,18 +2164,27 @@ int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff, cons
   if (ff->nrt <= 0 || ff->nlt <= 0 || !ff->rc || !ff->ac || !ff->ivor || !ff->emin || !ff->rmin2)
     return fail(ATJ_ERR_INVALID, "empty pair table or NULL pair-table array");
   if (rec->nrec < 0 || (rec->nrec > 0 && !rec->beads)) return fail(ATJ_ERR_INVALID, "receptor beads NULL");
+  if ((long long)ff->nrt * ff->nlt > 4096) return fail(ATJ_ERR_UNSUPPORTED, "pair table larger than 4096 entries");
   const float c2 = cutoff * cutoff;
   const float4* rb = reinterpret_cast<const float4*>(rec->beads);
+  if (rec->nrec == 0) return fail(ATJ_ERR_INVALID, "receptor has no beads");
+  const size_t lds = kBuildChunk * sizeof(float4) + (size_t)(2 * ff->nrt + 1 + kBuildThreads) * sizeof(int);
+  auto kern = build_potentials_kernel;
+  if (lds > 65536)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  const unsigned groups = (unsigned)((geom->ntypes + kBuildTypes - 1) / kBuildTypes);
   long long nv = (long long)G.dx * G.dy * G.dz;
-  hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
-                     (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2, ff->nrt,
-                     ff->nlt, lig_fftype_dev, c2, inner_out, G.dx, G.dy, G.dz, 0);
+  hipLaunchKernelGGL(kern, dim3((unsigned)((nv + kBuildThreads - 1) / kBuildThreads), groups), dim3(kBuildThreads),
+                     lds, (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2,
+                     ff->nrt, ff->nlt, lig_fftype_dev, geom->ntypes, c2, inner_out, G.dx, G.dy, G.dz, 0);
   HIPCHK(hipGetLastError());
   if (outer_out) {
     nv = (long long)G.dx2 * G.dy2 * G.dz2;
-    hipLaunchKernelGGL(build_potentials_kernel, dim3((unsigned)((nv + 255) / 256), geom->ntypes), dim3(256), 0,
-                       (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor, ff->emin, ff->rmin2,
-                       ff->nrt, ff->nlt, lig_fftype_dev, c2, outer_out, G.dx2, G.dy2, G.dz2, 1);
+    hipLaunchKernelGGL(kern, dim3((unsigned)((nv + kBuildThreads - 1) / kBuildThreads), groups),
+                       dim3(kBuildThreads), lds, (hipStream_t)stream, G, rb, rec->nrec, ff->rc, ff->ac, ff->ivor,
+                       ff->emin, ff->rmin2, ff->nrt, ff->nlt, lig_fftype_dev, geom->ntypes, c2, outer_out, G.dx2,
+                       G.dy2, G.dz2, 1);
     HIPCHK(hipGetLastError());
   }
   return ATJ_OK;

@@ -287,15 +287,17 @@ int twin_bead_terms(const TwinGrid* G, const TwinFF* ff, const float* rec4, cons
  * (test5.py:451-457; second moment of g - m_prev, plus eps_root); gravity adds
  * gravity |t|^2 to the energy (test5.py:418-420).  Bias corrections by powf as optax
  * computes 1 - b**count. */
+/* bias_pow = 0: bias corrections 1 - b^k by the recurrence c_k = b c_{k-1} + (1 - b), as the
+ * GPU minimizer computes them; 1: by powf (the two differ by rounding only). */
 int twin_minimize(const TwinGrid* G, const TwinFF* ff, const float* rec4, const float* lig4, int nb, float* pose7,
                   int N, int nsteps, float lr_rot, float lr_trans, float b1, float b2, float eps, int kind,
-                  float eps_root, float gravity, unsigned flags, float* energy, int nthreads) {
+                  float eps_root, float gravity, unsigned flags, float* energy, int nthreads, int bias_pow) {
   set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int p = 0; p < N; ++p) {
     float* P = pose7 + (size_t)p * 7;
     float m[7] = {0}, v[7] = {0};
-    float e = 0.f;
+    float e = 0.f, c1 = 0.f, c2 = 0.f;
     for (int k = 1; k <= nsteps; ++k) {
       float R[9], acc[13];
       quat_to_mat(P, R);
@@ -317,7 +319,9 @@ int twin_minimize(const TwinGrid* G, const TwinFF* ff, const float* rec4, const 
         e = e + (n * n) * gravity;
         for (int i = 4; i < 7; ++i) gr[i] = gr[i] + (2.f * gravity) * P[i];
       }
-      const float bc1 = 1.f - powf(b1, (float)k), bc2 = 1.f - powf(b2, (float)k);
+      c1 = fmaf(b1, c1, 1.f - b1);
+      c2 = fmaf(b2, c2, 1.f - b2);
+      const float bc1 = bias_pow ? 1.f - powf(b1, (float)k) : c1, bc2 = bias_pow ? 1.f - powf(b2, (float)k) : c2;
       for (int i = 0; i < 7; ++i) {
         const float pe = kind == 1 ? gr[i] - m[i] : gr[i];
         m[i] = (1.f - b1) * gr[i] + b1 * m[i];

@@ -116,8 +116,9 @@ class Twin:
         return out
 
     def minimize(self, pose7, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8, flags=None, nthreads=0,
-                 kind=0, eps_root=0.0, gravity=0.0):
-        """kind 0 = optax.adam, 1 = optax.adabelief; gravity = restraint weight."""
+                 kind=0, eps_root=0.0, gravity=0.0, bias_pow=False):
+        """kind 0 = optax.adam, 1 = optax.adabelief; gravity = restraint weight; bias_pow: bias
+        corrections by powf instead of the recurrence the GPU minimizer uses."""
         lib = load()
         P = np.array(pose7, np.float32, order="C").reshape(-1, 7)
         N = len(P)
@@ -128,5 +129,5 @@ class Twin:
                           ctypes.c_float(lr_rot), ctypes.c_float(lr_trans), ctypes.c_float(beta1),
                           ctypes.c_float(beta2), ctypes.c_float(eps), ctypes.c_int(int(kind)),
                           ctypes.c_float(eps_root), ctypes.c_float(gravity), ctypes.c_uint(fl), _p(E),
-                          ctypes.c_int(nthreads))
+                          ctypes.c_int(nthreads), ctypes.c_int(int(bool(bias_pow))))
         return P, E

@@ -89,19 +89,31 @@ def pose_rmsd(Pa, Pb, coor_lig):
     return O.rmsd(xa, xb)
 
 
-def twin_conditioning(twin, pose7, nsteps, lr_rot, lr_trans, coor_lig, ulps=4, tol=0.005, **kw):
-    """Minimize with the FP32 twin from the given starts and from starts whose translations are
-    `ulps` units in the last place larger.  A pose whose two results differ by more than `tol`
-    A RMSD amplifies rounding differences (the optimizer's normalised steps flip with the sign of
-    gradient components near zero): another FP32 implementation, whose sums round differently,
-    cannot be expected to follow it.  Returns (poses, energies, well_conditioned, rmsd)."""
+def _nudged(P, cols, ulps, direction):
+    Q = P.copy()
+    for _ in range(ulps):
+        Q[:, cols] = np.nextafter(Q[:, cols], np.float32(direction))
+    return Q
+
+
+def twin_conditioning(twin, pose7, nsteps, lr_rot, lr_trans, coor_lig, ulps=(1, 4, 16), tol=0.005, **kw):
+    """Minimize with the FP32 twin from the given starts, once more with its bias corrections
+    rounded otherwise at every step, and from nudged copies: the translations, then the
+    quaternion, each up and down by every count of units in the last place in `ulps`.  A pose whose nudged results differ from its own by more than `tol` A RMSD
+    amplifies rounding differences (the optimizer's normalised steps flip with the sign of
+    gradient components near zero, a bead crosses the inner grid's edge a step earlier or later):
+    another FP32 implementation, whose sums round differently, cannot be expected to follow it.
+    Returns (poses, energies, well_conditioned, largest nudged RMSD)."""
     P0 = np.array(pose7, np.float32).reshape(-1, 7)
     Pa, Ea = twin.minimize(P0, nsteps, lr_rot, lr_trans, **kw)
-    P1 = P0.copy()
-    for _ in range(ulps):
-        P1[:, 4:] = np.nextafter(P1[:, 4:], np.float32(np.inf))
-    Pb, _ = twin.minimize(P1, nsteps, lr_rot, lr_trans, **kw)
+    # and the same starts with the bias corrections rounded otherwise at every step
+    Pb, _ = twin.minimize(P0, nsteps, lr_rot, lr_trans, bias_pow=True, **kw)
     r = pose_rmsd(Pa, Pb, coor_lig)
+    for u in ulps:
+        for cols, direction in ((slice(4, 7), np.inf), (slice(4, 7), -np.inf), (slice(0, 4), np.inf),
+                                (slice(0, 4), -np.inf)):
+            Pb, _ = twin.minimize(_nudged(P0, cols, u, direction), nsteps, lr_rot, lr_trans, **kw)
+            r = np.maximum(r, pose_rmsd(Pa, Pb, coor_lig))
     return Pa, Ea, r < tol, r
 
 

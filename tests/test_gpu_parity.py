@@ -217,6 +217,14 @@ def test_minimize_vs_twin_and_oracle(s1):
     print(f"C1 minimization vs twin: {good.sum()} of {len(good)} poses well conditioned; RMSD max "
           f"{r[good].max():.2e} median {np.median(r[good]):.2e} A; energy rel max {_rel(Eg[good], Et[good]).max():.2e}"
           f"; all poses: RMSD median {np.median(r):.2e}, 99th pct {np.percentile(r, 99):.2e}")
+    bad = np.nonzero(good & (r >= 0.05))[0]
+    for i in bad:
+        xp = O.transform(O.mat_from_qt(P[i:i + 1, :4].astype(np.float64), P[i:i + 1, 4:].astype(np.float64)),
+                         lig.coor)[0]
+        v, inside = O.voxelize(xp, s1.grid.origin, s1.grid.gridspacing, s1.grid.dim)
+        edge = np.minimum(np.abs(v), np.abs(v - (np.asarray(s1.grid.dim) - 1))).min(axis=1)
+        print(f"  pose {i}: RMSD {r[i]:.3e} A, nudged {rc[i]:.2e}, E gpu {Eg[i]:.4f} twin {Et[i]:.4f}, beads inside "
+              f"{inside.tolist()}, voxels from the inner edge {np.round(edge, 3).tolist()}")
     assert good.mean() >= 0.9
     assert r[good].max() < 0.05
     # the energy follows the pose: where the poses agree to 1e-4 A, 99% of the energies agree to

@@ -16,7 +16,7 @@ from attract_jax_amd.protocol import ConformerSet, dock_test5, prepare, run_sche
 from oracle import oracle as O
 from oracle.twin import Twin
 
-from .systems import pose_rmsd, twin_conditioning
+from .systems import _nudged, pose_rmsd, twin_conditioning
 from .test_gpu_parity import Sys
 
 pytestmark = pytest.mark.gpu
@@ -80,11 +80,11 @@ def test_schedule_vs_twin_stage_by_stage(s1):
     assert np.all(st.count.cpu().numpy() == SHORT[-1][0])
     start = np.concatenate([w.quat[:n], w.trans[:n]], 1).astype(np.float32)
     Pt, Et = _twin_schedule(s1.twins[0], start, SHORT)
-    moved = start.copy()
-    for _ in range(4):
-        moved[:, 4:] = np.nextafter(moved[:, 4:], np.float32(np.inf))
-    Pm, _ = _twin_schedule(s1.twins[0], moved, SHORT)
-    good = pose_rmsd(Pt, Pm, w.ligands[0].coor) < 0.005
+    good = np.ones(n, bool)
+    for cols, direction in ((slice(4, 7), np.inf), (slice(4, 7), -np.inf), (slice(0, 4), np.inf),
+                            (slice(0, 4), -np.inf)):
+        Pm, _ = _twin_schedule(s1.twins[0], _nudged(start, cols, 4, direction), SHORT)
+        good &= pose_rmsd(Pt, Pm, w.ligands[0].coor) < 0.005
     r = _rmsd_to_twin(w.ligands[0].coor, st.pose.cpu().numpy(), Pt)
     E = st.energy.cpu().numpy()
     erel = np.abs(E - Et) / np.maximum(np.abs(Et), 1.0)
