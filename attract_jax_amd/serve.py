@@ -122,6 +122,7 @@ class Group:
     key: object
     grid_bytes: int
     pool: SlotPool
+    layout: object = None        # the grid layout the backend chose for it
     jobs: list = field(default_factory=list)
     inflight: object = None      # backend token of the running chunk
     chunk_jobs: list = field(default_factory=list)
@@ -146,6 +147,7 @@ class Scheduler:
         self.jobs = []
         self.done = []
         self.used = 0
+        self.evictions = 0
 
     # ------------------------------------------------------------------ bookkeeping
     def submit(self, spec, now=None):
@@ -162,6 +164,13 @@ class Scheduler:
         self.backend.drop_group(g)
         self.used -= g.grid_bytes + self._group_bytes(g.pool.capacity)
         del self.groups[g.key]
+        self.evictions += 1
+
+    def _grid_options(self, key, job):
+        """[(layout, grid bytes)] the backend offers for a new group, preferred first."""
+        if hasattr(self.backend, "grid_options"):
+            return self.backend.grid_options(key, job)
+        return [(None, self.backend.grid_bytes(key))]
 
     def _evict_for(self, need, keep=None):
         """Free idle resident grids (LRU first) until `need` more bytes fit."""
@@ -188,12 +197,12 @@ class Scheduler:
         while self.pending:
             job = self.pending[0]
             key = job.spec.receptor_key
-            gbytes = self.backend.grid_bytes(key)
-            if gbytes + self._group_bytes(max(self.initial_slots, job.npose)) > self.budget:
+            g = self.groups.get(key)
+            options = self._grid_options(key, job) if g is None else [(g.layout, g.grid_bytes)]
+            if min(b for _, b in options) + self._group_bytes(max(self.initial_slots, job.npose)) > self.budget:
                 self.pending.popleft()
                 self._fail(job, "job does not fit the HBM budget")
                 continue
-            g = self.groups.get(key)
             start = -1
             if g is not None:
                 start = g.pool.alloc(job.npose)
@@ -221,10 +230,13 @@ class Scheduler:
                     start = g.pool.alloc(job.npose)
             if g is None:
                 cap = max(self.initial_slots, job.npose)
-                need = gbytes + self._group_bytes(cap)
-                if not self._evict_for(need):
+                for layout, gbytes in options:  # the first layout that fits, evicting idle grids for it
+                    need = gbytes + self._group_bytes(cap)
+                    if need <= self.budget and self._evict_for(need):
+                        break
+                else:
                     return  # wait until running jobs free memory
-                g = Group(key, gbytes, SlotPool(cap))
+                g = Group(key, gbytes, SlotPool(cap), layout)
                 self.backend.make_group(g)
                 self.used += need
                 self.groups[key] = g
@@ -307,42 +319,82 @@ class Scheduler:
     def latencies(self):
         return np.array([j.t_done - j.t_arrival for j in self.done if j.state == "done"])
 
+    def admission_waits(self):
+        return np.array([j.t_admit - j.t_arrival for j in self.done if j.state == "done"])
+
 
 # ---------------------------------------------------------------------------- GPU backend
 
 class GpuBackend:
-    """Drives the HIP kernels: resident grids, per-receptor slot arrays, one stream per group."""
+    """Drives the HIP kernels: resident grids, per-receptor slot arrays, one stream per group.
 
-    def __init__(self, device, receptors, layout=None, waves_per_block=16, nstreams=4):
+    receptors: key -> workloads.ReceptorSpec (the grid is built on the device when a group for
+    the receptor is made, on the group's stream) or a dict {'grid_bytes', 'build'} whose build()
+    returns a ScoringSystem.  The grid layout of a ReceptorSpec is chosen per group: corner-packed
+    (8x the plain footprint, about twice the minimizer's speed) for jobs of at least
+    corner_min_work pose-steps when it fits, plain otherwise; layout= forces one."""
+
+    def __init__(self, device, receptors, layout=None, waves_per_block=16, nstreams=4, corner_min_work=2e6,
+                 launch_options=0):
         import torch
-        from . import _lib
         self.torch = torch
         self.device = torch.device(device)
-        self.receptors = receptors    # key -> callable returning (ScoringSystem) when built
-        self.layout = layout if layout is not None else _lib.ATJ_LAYOUT_CORNER
+        self.receptors = receptors
+        self.layout = layout
+        self.corner_min_work = float(corner_min_work)
         self.waves = waves_per_block
+        self.options = launch_options
         self.streams = [torch.cuda.Stream(self.device) for _ in range(nstreams)]
         self._next_stream = 0
         self._built = {}
+        self.build_seconds = []       # host time of each grid build (its launches and syncs)
+        self.layouts = []             # layout of each group made
 
     def grid_bytes(self, key):
-        return self.receptors[key]["grid_bytes"]
+        return self.grid_options(key, None)[0][1]
 
-    def _system(self, key):
+    def grid_options(self, key, job):
+        from . import _lib
+        r = self.receptors[key]
+        if isinstance(r, dict):
+            return [(self.layout, r["grid_bytes"])]
+        lists = r.list_bytes()
+        corner = (_lib.ATJ_LAYOUT_CORNER, r.corner_bytes() + lists)
+        plain = (_lib.ATJ_LAYOUT_PLAIN, r.plain_bytes() + lists)
+        if self.layout is not None:
+            return [corner if self.layout == _lib.ATJ_LAYOUT_CORNER else plain]
+        work = 0 if job is None else job.npose * job.spec.nsteps
+        return [corner, plain] if work >= self.corner_min_work else [plain]
+
+    def _system(self, key, layout=None):
         if key not in self._built:
-            self._built[key] = self.receptors[key]["build"]()
+            r = self.receptors[key]
+            if isinstance(r, dict):
+                self._built[key] = r["build"]()
+            else:
+                from .device import DeviceGrid, ForceField, Receptor
+                from .score import ScoringSystem
+                from .workloads import NTYPES
+                ff = ForceField.create(r.rc, r.ac, r.ivor, device=self.device)
+                rec = Receptor.create(r.coor, r.types - 1, self.device)
+                grid = DeviceGrid.build(rec, ff, np.arange(NTYPES), layout=layout, device=self.device, **r.grid_kw)
+                self._built[key] = ScoringSystem(grid, ff, rec)
         return self._built[key]
 
     def make_group(self, g):
         from . import _lib
         from .minimize import PoseState
         torch = self.torch
-        system = self._system(g.key)
-        system.launch = _lib.AtjLaunch(self.waves, 0, 0)
         cap = g.pool.capacity
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
-        stream.wait_stream(torch.cuda.current_stream(self.device))  # grid built on the current stream
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):  # the grid is built on the group's stream
+            system = self._system(g.key, g.layout)
+        self.build_seconds.append(time.perf_counter() - t0)
+        self.layouts.append(system.grid.layout)
+        system.launch = _lib.AtjLaunch(self.waves, 0, self.options)
+        stream.wait_stream(torch.cuda.current_stream(self.device))  # a grid built beforehand
         # the group's arrays belong to its stream: the caching allocator then reuses their
         # memory only in that stream's order once they are dropped (grow_group, drop_group)
         z = torch.zeros

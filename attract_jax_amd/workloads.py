@@ -42,6 +42,24 @@ def random_receptor(n, radius, min_sep, rng, max_tries=200000):
     return pts
 
 
+def jittered_receptor(n, radius, min_sep, rng):
+    """n points inside a sphere, pairwise >= min_sep apart: random sites of a cubic lattice (as
+    dense as the sphere needs for n sites, with room to spare) each moved by a uniform jitter
+    that keeps neighbours min_sep apart.  Vectorised: for the many receptors of a serving
+    stream, where sequential addition would take seconds each."""
+    a = (4.0 / 3.0 * np.pi * radius ** 3 / (1.3 * n)) ** (1.0 / 3.0)
+    if a < min_sep:
+        raise ValueError("too many beads for the sphere at this separation")
+    k = int(np.ceil(radius / a)) + 1
+    g = np.stack(np.meshgrid(*[np.arange(-k, k + 1)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    j = (a - min_sep) / 2.0
+    g = g[np.linalg.norm(g, axis=1) <= radius - j * np.sqrt(3.0)]
+    if len(g) < n:
+        raise ValueError("lattice too coarse for n beads")
+    pts = g[rng.choice(len(g), n, replace=False)]
+    return pts + rng.uniform(-j, j, size=pts.shape)
+
+
 def random_pair_table(rng, ntypes=NTYPES, p_repulsive=0.1):
     """(rc, ac, ivor, rmin, emin_abs) symmetric ntypes x ntypes float64 tables."""
     iu = np.triu_indices(ntypes)
@@ -212,6 +230,82 @@ def build_system(w, device=None, layout=None, keep_plain=False, flags=None, laun
     ligs = [Ligand.create(l.coor, l.types - 1, l.types - 1, device) for l in w.ligands]
     kw = {} if nshell is None else dict(nshell=nshell)
     return ScoringSystem(grid, ff, rec, flags=flags, launch=launch, **kw), ligs
+
+
+@dataclass
+class ReceptorSpec:
+    """A serving job's receptor: beads, types 1..32, its own pair table and grid geometry."""
+    key: object
+    coor: np.ndarray
+    types: np.ndarray
+    rc: np.ndarray
+    ac: np.ndarray
+    ivor: np.ndarray
+    grid_kw: dict
+
+    @property
+    def radius(self):
+        return float(np.linalg.norm(self.coor, axis=1).max())
+
+    def plain_bytes(self):
+        d, d2 = self.grid_kw["dim"], self.grid_kw["dim2"]
+        return NTYPES * 16 * (int(np.prod(d)) + int(np.prod(d2)))
+
+    def corner_bytes(self):
+        d, d2 = self.grid_kw["dim"], self.grid_kw["dim2"]
+        return NTYPES * 128 * (int(np.prod(np.subtract(d, 1))) + int(np.prod(np.subtract(d2, 1))))
+
+    def list_bytes(self):
+        """Neighbour-list memory, estimated from the bead density (lists of about 4/3 pi
+        neighbourdis^3 x density entries near the receptor, padded to 32), plus the per-point
+        records."""
+        v = int(np.prod(self.grid_kw["dim"]))
+        r = self.radius + self.grid_kw["neighbourdis"]
+        near = min(1.0, 4.0 / 3.0 * np.pi * r ** 3 / (v * self.grid_kw["spacing"] ** 3))
+        dens = len(self.coor) / (4.0 / 3.0 * np.pi * max(self.radius, 1.0) ** 3)
+        per = 4.0 / 3.0 * np.pi * self.grid_kw["neighbourdis"] ** 3 * dens
+        return int(v * (8 + 16) + v * near * 2 * 32 * np.ceil(per / 32 + 1))
+
+
+def c5_receptor(key, nrec, rng):
+    """Receptor of a serving job: nrec beads at the C2 bead density (radius 25 A x (nrec/2000)^(1/3)),
+    own 32-type pair table, inner grid at 0.9 A reaching 12-26 A beyond the receptor (more for
+    larger receptors), outer grid at 1.8 A another 14.4 A out: plain fp32 grids of 0.18 GB (500
+    beads) to 1.4 GB (4000 beads)."""
+    radius = 25.0 * (nrec / 2000.0) ** (1.0 / 3.0)
+    coor = jittered_receptor(nrec, radius, 2.25, rng)
+    types = rng.integers(1, NTYPES + 1, size=nrec)
+    rc, ac, ivor, _, _ = random_pair_table(rng)
+    margin = 12.0 + 14.0 * (nrec - 500) / 3500.0
+    d = int(np.ceil(2 * (radius + margin) / 0.9)) + 1
+    d2 = (d + 1) // 2 + 16
+    o = -0.9 * (d - 1) / 2
+    grid_kw = dict(origin=(o, o, o), spacing=0.9, dim=(d, d, d), dim2=(d2, d2, d2), gridextension=16,
+                   plateaudis=10.0, neighbourdis=10.8)
+    return ReceptorSpec(key, coor, types, rc, ac, ivor, grid_kw)
+
+
+def c5_stream(njobs=200, rate=20.0, seed=5999, pose_scale=1.0, nrec_range=(500, 4001)):
+    """The serving workload: Poisson arrivals (mean `rate` jobs/s); each job brings its own
+    receptor (500-4000 beads, its own pair table and grid), a ligand of 5-60 beads, log-uniform
+    1e3..2e5 poses (times pose_scale) around the receptor surface and 100-300 ADAM steps
+    (lr 1e-3).  Returns the JobSpecs (receptor_key = the job's receptor) and {key: ReceptorSpec}."""
+    from .serve import JobSpec
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    specs, receptors = [], {}
+    for i in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        r = c5_receptor(("c5", i), int(rng.integers(*nrec_range)), np.random.default_rng(seed + 7000 + i))
+        receptors[r.key] = r
+        nb = int(rng.integers(5, 61))
+        coor = random_chain(nb, np.random.default_rng(seed + 1000 + i))
+        types = rng.integers(1, NTYPES + 1, size=nb)
+        npose = max(1, int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))) * pose_scale))
+        rad = r.radius
+        q, tr = posemod.random_poses(npose, np.random.default_rng(seed + 5000 + i), rad, rad + 10.0)
+        specs.append(JobSpec(r.key, coor, types, q, tr, int(rng.integers(100, 301)), arrival=t))
+    return specs, receptors
 
 
 def c5_receptors(n=16, seed=5000):

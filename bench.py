@@ -174,7 +174,8 @@ def main():
     ap.add_argument("--launch-options", type=int, default=0, help="ATJ_LAUNCH_* bits when --waves is given")
     ap.add_argument("--nshell", type=int, default=6, help="distance shells of the neighbour lists (0 = whole lists)")
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the serving run (0 = skip)")
-    ap.add_argument("--serve-receptors", type=int, default=16)
+    ap.add_argument("--serve-budget-gib", type=float, default=48.0,
+                    help="HBM the serving loop may hold (grids + pose slots)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean job arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (scale reference for FETCH_SIZE / WRITE_SIZE counters)")
@@ -387,30 +388,36 @@ def main():
         from attract_jax_amd import workloads as W
         from attract_jax_amd.serve import GpuBackend, Scheduler, run_stream
         t0 = time.time()
-        rs = W.c5_receptors(args.serve_receptors)
-        specs = W.c5_jobs(rs, args.serve_jobs, rate=args.serve_rate)
-        log(f"[bench] serving workload generated in {time.time() - t0:.1f}s")
-        be = GpuBackend(dev, W.c5_backend_receptors(rs, dev), waves_per_block=system.launch.waves_per_block)
-        t0 = time.time()
-        for r in rs:  # grids are precomputed: build them all before the arrival clock starts
-            be._system(r["key"])
-        torch.cuda.synchronize()
-        tb = time.time() - t0
-        log(f"[bench] {len(rs)} receptor grids built in {tb:.1f}s")
-        sch = Scheduler(be, hbm_budget=200 * 2 ** 30, chunk_steps=25)
-        lat, makespan, ps = run_stream(sch, specs, timeout=300)
-        nposes = [len(s.quat) for s in specs]
+        specs, receptors = W.c5_stream(args.serve_jobs, rate=args.serve_rate)
+        log(f"[bench] serving workload generated in {time.time() - t0:.1f}s ({len(receptors)} receptors)")
+        be = GpuBackend(dev, receptors, waves_per_block=system.launch.waves_per_block,
+                        launch_options=system.launch.options & _lib.ATJ_LAUNCH_CLAIM_MASK)
+        budget = int(args.serve_budget_gib * 2 ** 30)
+        sch = Scheduler(be, hbm_budget=budget, chunk_steps=25)
+        lat, makespan, ps = run_stream(sch, specs, timeout=600)
+        waits = sch.admission_waits()
+        nposes = [len(sp.quat) for sp in specs]
+        plain_gb = [r.plain_bytes() / 1e9 for r in receptors.values()]
+        corner = sum(1 for lay in be.layouts if lay == _lib.ATJ_LAYOUT_CORNER)
         result["serving"] = {
-            "jobs": len(specs), "completed": int((np.array([j.state for j in sch.done]) == "done").sum()),
-            "arrival_rate_jobs_per_s": args.serve_rate, "receptors": len(rs),
+            "workload": "C5: Poisson arrivals, every job with its own receptor (500-4000 beads, own pair table) "
+                        "whose grid is built on the device at admission, ligand 5-60 beads, 1e3-2e5 poses, "
+                        "100-300 ADAM steps; latency = completion - arrival",
+            "jobs": len(specs), "completed": int(sum(j.state == "done" for j in sch.done)),
+            "arrival_rate_jobs_per_s": args.serve_rate, "hbm_budget_gib": args.serve_budget_gib,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "mean_latency_s": float(lat.mean()), "makespan_s": makespan, "pose_steps": ps,
-            "aggregate_pose_steps_per_s": ps / makespan, "grid_build_s": tb,
-            "poses_per_job": [int(min(nposes)), int(max(nposes))], "chunk_steps": sch.chunk_steps,
-            "note": "jobs = (receptor from a pool of precomputed resident grids, ligand 5-60 beads, 1e3-2e5 poses, "
-                    "100-300 ADAM steps), Poisson arrivals; latency = completion - arrival"}
+            "mean_latency_s": float(lat.mean()), "p50_admission_wait_s": float(np.percentile(waits, 50)),
+            "p99_admission_wait_s": float(np.percentile(waits, 99)), "evictions": sch.evictions,
+            "groups_corner_layout": corner, "groups_plain_layout": len(be.layouts) - corner,
+            "grid_build_s_p50": float(np.percentile(be.build_seconds, 50)),
+            "grid_build_s_max": float(np.max(be.build_seconds)),
+            "plain_grid_gb": [round(min(plain_gb), 3), round(max(plain_gb), 3)],
+            "makespan_s": makespan, "pose_steps": ps, "aggregate_pose_steps_per_s": ps / makespan,
+            "poses_per_job": [int(min(nposes)), int(max(nposes))], "chunk_steps": sch.chunk_steps}
         log(f"[bench] serving: p50 {result['serving']['p50_latency_s']:.3f}s p99 "
-            f"{result['serving']['p99_latency_s']:.3f}s, {ps / makespan:.3e} pose-steps/s over {makespan:.1f}s")
+            f"{result['serving']['p99_latency_s']:.3f}s, admission wait p99 "
+            f"{result['serving']['p99_admission_wait_s']:.3f}s, {sch.evictions} evictions, {corner} corner / "
+            f"{len(be.layouts) - corner} plain grids, {ps / makespan:.3e} pose-steps/s over {makespan:.1f}s")
 
     print(json.dumps(result), flush=True)
 

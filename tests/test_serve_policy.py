@@ -287,3 +287,39 @@ def test_job_the_backend_cannot_load_fails_without_taking_slots():
     drive(s, clock)
     assert bad.state == "failed" and "bead" in bad.error
     assert good.state == "done" and s.groups[0].pool.capacity == 16
+
+
+class TwoLayouts(FakeBackend):
+    """Offers a large preferred layout and a small fallback for every receptor."""
+
+    def grid_options(self, key, job):
+        return [("big", 4000), ("small", 500)]
+
+    def make_group(self, g):
+        g.handle = {"layout": g.layout}
+
+
+def test_layout_falls_back_when_the_preferred_one_does_not_fit():
+    clock = FakeClock()
+    be = TwoLayouts(clock, {})
+    s = Scheduler(be, hbm_budget=5000 + 64 * 100 * 2, chunk_steps=10, initial_slots=64, clock=clock)
+    a = s.submit(spec(0, 10, 200))
+    s.tick()
+    b = s.submit(spec(1, 10, 10))  # receptor 0 holds 4000 B while it runs: 1 gets the small layout
+    s.tick()
+    assert s.groups[0].layout == "big" and s.groups[1].layout == "small"
+    drive(s, clock)
+    assert a.state == b.state == "done"
+
+
+def test_evictions_and_admission_waits_are_counted():
+    clock = FakeClock()
+    be = FakeBackend(clock, {k: 1000 for k in range(4)})
+    per = 64 * 100
+    s = Scheduler(be, hbm_budget=2 * (1000 + per), chunk_steps=10, initial_slots=64, clock=clock)
+    jobs = [s.submit(spec(k, 10, 30)) for k in range(4)]
+    drive(s, clock)
+    assert all(j.state == "done" for j in jobs)
+    assert s.evictions >= 2
+    waits = s.admission_waits()
+    assert len(waits) == 4 and waits.max() > 0
