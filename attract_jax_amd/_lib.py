@@ -22,6 +22,9 @@ ATJ_POSE_QT = 0x10
 ATJ_GRAD_QT = 0x20
 
 ATJ_NSTATS = 4
+ATJ_NPHASES = 6
+PHASES = ("set-up and claims", "bead phase", "segment-size scans", "pair phase", "reductions",
+          "optimizer update")
 
 c_float_p = ctypes.c_void_p  # device pointers are passed as integers
 

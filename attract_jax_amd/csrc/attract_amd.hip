@@ -539,6 +539,31 @@ struct KArgs {
   unsigned long long* stats;
 };
 
+// Per-wavefront launch statistics: the ATJ_NSTATS counters and, in a build with
+// ATJ_PHASE_CLOCKS defined, the shader-clock cycles the wavefront spent in each phase of the
+// evaluation (claims and set-up, bead phase, segment-size scans, pair phase, reductions,
+// optimizer update), accumulated into stats[ATJ_NSTATS + phase].
+enum Phase { kPhSetup = 0, kPhBead, kPhScan, kPhPair, kPhReduce, kPhUpdate, kNumPhases };
+static_assert(kNumPhases == ATJ_NPHASES, "phase count of the header");
+struct WaveStats {
+  unsigned n[ATJ_NSTATS];
+#ifdef ATJ_PHASE_CLOCKS
+  unsigned long long clk[kNumPhases];
+  unsigned long long last;
+#endif
+};
+
+__device__ __forceinline__ void phase_end(WaveStats& st, int phase) {
+#ifdef ATJ_PHASE_CLOCKS
+  const unsigned long long now = __builtin_amdgcn_s_memtime();
+  st.clk[phase] += now - st.last;
+  st.last = now;
+#else
+  (void)st;
+  (void)phase;
+#endif
+}
+
 // Far-away receptor bead standing for list padding (0xFFFF entries): every pair with it
 // lies beyond the plateau and uses the all-zero pair record.
 __device__ __forceinline__ float4 sentinel_bead() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
@@ -719,7 +744,7 @@ struct PoseIn {
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws, const PoseIn& A,
-                              const PoseIn& B, unsigned st[4], int lane, float& uA, float& uB) {
+                              const PoseIn& B, WaveStats& st, int lane, float& uA, float& uB) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
@@ -731,6 +756,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
   uA = uB = 0.f;
+  phase_end(st, kPhSetup);
 
   // a pose without beads has energy and gradient 0; paired with B it still runs B's chunk
   const int nloop = max(A.nb, nposes == 2 ? B.nb : 0);
@@ -764,12 +790,13 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       unsigned in_any;
       bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
       if (MODE != kAdjoint) bead_acc(acc, Eg, g, xl);
-      st[0] += in_any;
+      st.n[0] += in_any;
       if (voxel >= 0) {
-        st[1] += 1;
-        st[2] += (unsigned)cnt;
+        st.n[1] += 1;
+        st.n[2] += (unsigned)cnt;
       }
     }
+    phase_end(st, kPhBead);
     if (nposes == 2) {  // B's bead terms wait in LDS; its lanes start A's pair sums from zero
       if (h) ws->gridB[lane - kHalf] = make_float4(Eg, g[0], g[1], g[2]);
 #pragma unroll
@@ -800,6 +827,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       ws->xl[lane] = make_float4(-2.f * xl[0], -2.f * xl[1], -2.f * xl[2], __int_as_float(start));
     }
     wave_sync();
+    phase_end(st, kPhScan);
     unsigned npair = 0;  // wave-wide count, credited to lane 0
     for (int q = 0; q < nposes; ++q) {
       if (q == 1 && MODE != kAdjoint) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
@@ -843,7 +871,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
         pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, n4, tail4 ? seg4t : sq,
                                                      tail4 ? excl4t : excl, wgt, acc, npair, lane);
     }
-    st[3] += lane == 0 ? npair : 0u;
+    phase_end(st, kPhPair);
+    st.n[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
   if (MODE == kAdjoint) return;  // the parameter derivatives went to the wavefront's table
@@ -852,6 +881,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     uB = u;
   else
     uA = u;
+  phase_end(st, kPhReduce);
 }
 
 // Energy and dE/dM of pose p from the transposed sums: one lane per output word.
@@ -1068,7 +1098,7 @@ __device__ __forceinline__ long long first_claim(const KArgs& a, int lane) {
 // different jobs and have different step counts.
 template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              const float2* __restrict__ s_side, WaveScratch* ws, BatchScratch* bs, unsigned st[4],
+                              const float2* __restrict__ s_side, WaveScratch* ws, BatchScratch* bs, WaveStats& st,
                               int lane) {
   // Slot lanes of the optimizer update: lane 8 b + i updates parameter i (< 7) of slot b.
   const int sb = lane >> 3, pi = lane & 7;
@@ -1113,6 +1143,7 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
     const int src = sb < nbatch ? sb : 0;  // the own lane of this lane's slot
     const bool slot = sb < nbatch && pi < 7;
     wave_sync();
+    phase_end(st, kPhSetup);
     for (int s = 0; s < nst_max; ++s) {
       // active slots in order; two consecutive ones of at most kHalf beads share the bead phase
       for (int b = 0; b < nbatch; ++b) {
@@ -1175,6 +1206,7 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
       wave_sync();
       if (act) pose12_elements(bs->prm[sb], bs->M[sb], pi);
       wave_sync();
+      phase_end(st, kPhUpdate);
     }
     if (own) {
 #pragma unroll
@@ -1214,7 +1246,10 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
   float* w_pgrad = MODE == kAdjoint ? s_pgrad + wave * 3 * nslots : nullptr;
-  unsigned st[4] = {0, 0, 0, 0};
+  WaveStats st = {};
+#ifdef ATJ_PHASE_CLOCKS
+  st.last = __builtin_amdgcn_s_memtime();
+#endif
 
   if (MODE == kMinimize) {
     BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
@@ -1288,10 +1323,16 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
   }
   if (a.stats) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const unsigned s = wave_sum_u(st[i]);
+    for (int i = 0; i < ATJ_NSTATS; ++i) {
+      const unsigned s = wave_sum_u(st.n[i]);
       if (lane == 0 && s) atomicAdd(a.stats + i, (unsigned long long)s);
     }
+#ifdef ATJ_PHASE_CLOCKS
+    phase_end(st, kPhSetup);
+#pragma unroll
+    for (int i = 0; i < kNumPhases; ++i)
+      if (lane == 0) atomicAdd(a.stats + ATJ_NSTATS + i, st.clk[i]);
+#endif
   }
   if (MODE == kAdjoint) {  // the block's row of partial sums: wavefront tables added in order
     __syncthreads();

@@ -141,7 +141,7 @@ class ScoringSystem:
         self.flags = flags if flags is not None else _default_flags(self.grid, ("grid", "nb"))
         self.launch = launch or _lib.AtjLaunch(8, 0, 0)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
-        self.stats = torch.zeros(_lib.ATJ_NSTATS, dtype=torch.int64, device=self.device)
+        self.stats = torch.zeros(_lib.ATJ_NSTATS + _lib.ATJ_NPHASES, dtype=torch.int64, device=self.device)
         self.refresh()
 
     def refresh(self):

@@ -184,7 +184,12 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes that measure the minimizer's HBM traffic and VALU "
                          "instructions")
+    ap.add_argument("--phase-clocks", action="store_true",
+                    help="load the library built with ATJ_PHASE_CLOCKS, run one minimizer launch and print the "
+                         "share of wavefront time in each phase (no timed launches)")
     args = ap.parse_args()
+    if args.phase_clocks:
+        os.environ["ATTRACT_AMD_LIB"] = os.path.join(ROOT, "attract_jax_amd", "libattract_amd_clocks.so")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (poses shard trivially across "
                                    "GPUs with a replicated grid, but no multi-GPU path is built)"}))
@@ -248,6 +253,15 @@ def main():
     torch.cuda.synchronize()
     stats = system.stats.cpu().numpy().astype(np.float64)
     E_final = st.energy.cpu().numpy()
+    if args.phase_clocks:
+        clk = stats[_lib.ATJ_NSTATS:]
+        split = {name: float(c / clk.sum()) for name, c in zip(_lib.PHASES, clk)}
+        print(json.dumps({"phase_split": split, "wavefront_clock_cycles": float(clk.sum()),
+                          "workload": w.name, "layout": int(system.grid.layout),
+                          "waves_per_block": system.launch.waves_per_block,
+                          "options": system.launch.options}), flush=True)
+        return
+    stats = stats[:_lib.ATJ_NSTATS]
 
     ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]

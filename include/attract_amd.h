@@ -142,8 +142,13 @@ typedef struct AtjJob {
 
 /* Optional per-launch counters (device uint64[4]): beads inside a grid,
  * beads inside the inner grid, listed neighbour entries (sum of K_v),
- * evaluated pairs (dsq < plateaudis^2).  Accumulated, not reset. */
+ * evaluated pairs (dsq < plateaudis^2).  Accumulated, not reset.  A library built with
+ * ATJ_PHASE_CLOCKS defined also accumulates, in stats[ATJ_NSTATS + i], the shader-clock
+ * cycles its wavefronts spent in phase i (ATJ_NPHASES: claims and set-up, bead phase,
+ * segment-size scans, pair phase, reductions, optimizer update); stats then needs
+ * ATJ_NSTATS + ATJ_NPHASES entries. */
 #define ATJ_NSTATS 4
+#define ATJ_NPHASES 6
 
 /* Pose-state arrays of the minimizer (all device, indexed by pose slot). */
 typedef struct AtjPoseState {
