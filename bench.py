@@ -80,7 +80,34 @@ def scoring_rates(dev, wpb):
     return {"c2_energrad_poses_per_s": len(w2.quat) / (ms2 * 1e-3), "c2_energrad_ms": ms2, "c2_poses": len(w2.quat),
             "c4_adjoint_poses_per_s": len(w4.quat) / (ms4 * 1e-3), "c4_adjoint_ms": ms4, "c4_poses": len(w4.quat),
             "note": "energies + dE/dM of every pose (C2); sum over poses of dE/d(rc, ac, emin) with its partial-sum "
-                    "reduction (C4); best of 3 launches"}
+                    "reduction (C4); best of 3 launches",
+            "grid_build": grid_build_times(dev, best_ms, stream)}
+
+
+def grid_build_times(dev, best_ms, stream):
+    """The device grid builder on serving receptors of 500, 2000 and 4000 beads (32 types, inner
+    grid at 0.9 A, outer grid at 1.8 A): potentials and neighbour lists, plain and corner-packed,
+    and the lists alone; best of 3, HIP events on the build stream (the list allocation reads
+    the list total back to the host once per build, which the time includes)."""
+    from attract_jax_amd import _lib
+    from attract_jax_amd.device import DeviceGrid, ForceField, Receptor
+    from attract_jax_amd.workloads import NTYPES, c5_receptor
+
+    out = []
+    for nrec in (500, 2000, 4000):
+        r = c5_receptor(("build", nrec), nrec, np.random.default_rng(nrec))
+        ff = ForceField.create(r.rc, r.ac, r.ivor, device=dev)
+        rec = Receptor.create(r.coor, r.types - 1, dev)
+        row = {"receptor_beads": nrec, "inner_dim": r.grid_kw["dim"][0], "outer_dim": r.grid_kw["dim2"][0]}
+        for name, kw in (("plain_ms", dict(layout=_lib.ATJ_LAYOUT_PLAIN)),
+                         ("corner_ms", dict(layout=_lib.ATJ_LAYOUT_CORNER)),
+                         ("lists_only_ms", dict(layout=_lib.ATJ_LAYOUT_PLAIN, potentials=False))):
+            row[name] = best_ms(lambda: DeviceGrid.build(rec, ff, np.arange(NTYPES), device=dev, **kw, **r.grid_kw),
+                                stream)
+        log(f"[bench] grid build {nrec} beads, inner {row['inner_dim']}^3: plain {row['plain_ms']:.1f} ms, "
+            f"corner {row['corner_ms']:.1f} ms, lists only {row['lists_only_ms']:.1f} ms")
+        out.append(row)
+    return out
 
 
 def _child_command(args, wpb, opt):
