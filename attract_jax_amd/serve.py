@@ -329,13 +329,12 @@ class GpuBackend:
     """Drives the HIP kernels: resident grids, per-receptor slot arrays, one stream per group.
 
     receptors: key -> workloads.ReceptorSpec (the grid is built on the device when a group for
-    the receptor is made, on the group's stream) or a dict {'grid_bytes', 'build'} whose build()
+    the receptor is made) or a dict {'grid_bytes', 'build'} whose build()
     returns a ScoringSystem.  The grid layout of a ReceptorSpec is chosen per group: corner-packed
     (8x the plain footprint, about twice the minimizer's speed) for jobs of at least
     corner_min_work pose-steps when it fits, plain otherwise; layout= forces one."""
 
-    def __init__(self, device, receptors, layout=None, waves_per_block=16, nstreams=4, corner_min_work=2e6,
-                 launch_options=0):
+    def __init__(self, device, receptors, layout=None, waves_per_block=16, corner_min_work=2e6, launch_options=0):
         import torch
         self.torch = torch
         self.device = torch.device(device)
@@ -344,8 +343,12 @@ class GpuBackend:
         self.corner_min_work = float(corner_min_work)
         self.waves = waves_per_block
         self.options = launch_options
-        self.streams = [torch.cuda.Stream(self.device) for _ in range(nstreams)]
-        self._next_stream = 0
+        # One stream per group, so that a host wait on a group's stream (result copy, plan
+        # upload) waits for that group's work only; streams of dropped groups are reused.  Grids
+        # are built on the device's current stream, which no group launches on: the caching
+        # allocator keeps memory per stream, so grids built there share one pool of cached
+        # memory however many group streams there are.
+        self._free_streams = []
         self._built = {}
         self.build_seconds = []       # host time of each grid build (its launches and syncs)
         self.layouts = []             # layout of each group made
@@ -386,15 +389,13 @@ class GpuBackend:
         from .minimize import PoseState
         torch = self.torch
         cap = g.pool.capacity
-        stream = self.streams[self._next_stream % len(self.streams)]
-        self._next_stream += 1
+        stream = self._free_streams.pop() if self._free_streams else torch.cuda.Stream(self.device)
         t0 = time.perf_counter()
-        with torch.cuda.stream(stream):  # the grid is built on the group's stream
-            system = self._system(g.key, g.layout)
+        system = self._system(g.key, g.layout)
         self.build_seconds.append(time.perf_counter() - t0)
         self.layouts.append(system.grid.layout)
         system.launch = _lib.AtjLaunch(self.waves, 0, self.options)
-        stream.wait_stream(torch.cuda.current_stream(self.device))  # a grid built beforehand
+        stream.wait_stream(torch.cuda.current_stream(self.device))  # the grid's build
         # the group's arrays belong to its stream: the caching allocator then reuses their
         # memory only in that stream's order once they are dropped (grow_group, drop_group)
         z = torch.zeros
@@ -432,9 +433,13 @@ class GpuBackend:
         g.handle["plan"] = None
 
     def drop_group(self, g):
+        """Release an idle group (no launch or copy of it is pending, so its grid is free for
+        the next build at once).  The memory stays with the caching allocator for the next
+        grids: emptying the cache would wait for every running launch and make the next grid
+        a fresh device allocation."""
         self._built.pop(g.key, None)
+        self._free_streams.append(g.handle["stream"])
         g.handle = None
-        self.torch.cuda.empty_cache()
 
     def load_job(self, g, job, slot):
         """Device ligand and starting poses of a job in slots [slot, slot + npose).  ValueError

@@ -435,7 +435,34 @@ def main():
                         launch_options=system.launch.options & _lib.ATJ_LAUNCH_CLAIM_MASK)
         budget = int(args.serve_budget_gib * 2 ** 30)
         sch = Scheduler(be, hbm_budget=budget, chunk_steps=25)
+        # host time of each scheduler tick (admission builds grids on the host thread) and the
+        # caching allocator's retries (a retry frees every cached block, synchronizing the device)
+        ticks, tick = [], sch.tick
+
+        def timed_tick(now=None):
+            t = time.perf_counter()
+            tick(now)
+            ticks.append(time.perf_counter() - t)
+
+        sch.tick = timed_tick
+        host_ops = {}
+
+        def timed_op(name, fn):
+            def run(*a, **k):
+                t = time.perf_counter()
+                r = fn(*a, **k)
+                d = time.perf_counter() - t
+                n, tot, mx = host_ops.get(name, (0, 0.0, 0.0))
+                host_ops[name] = (n + 1, tot + d, max(mx, d))
+                return r
+            return run
+
+        for name in ("make_group", "load_job", "launch", "fetch_job", "drop_group", "grow_group", "reset_group"):
+            setattr(be, name, timed_op(name, getattr(be, name)))
+        mem0 = torch.cuda.memory_stats(dev)
+        torch.cuda.reset_peak_memory_stats(dev)
         lat, makespan, ps = run_stream(sch, specs, timeout=600)
+        mem1 = torch.cuda.memory_stats(dev)
         waits = sch.admission_waits()
         nposes = [len(sp.quat) for sp in specs]
         plain_gb = [r.plain_bytes() / 1e9 for r in receptors.values()]
@@ -452,13 +479,22 @@ def main():
             "groups_corner_layout": corner, "groups_plain_layout": len(be.layouts) - corner,
             "grid_build_s_p50": float(np.percentile(be.build_seconds, 50)),
             "grid_build_s_max": float(np.max(be.build_seconds)),
+            "tick_s_p99": float(np.percentile(ticks, 99)), "tick_s_max": float(np.max(ticks)),
+            "allocator_retries": int(mem1.get("num_alloc_retries", 0) - mem0.get("num_alloc_retries", 0)),
+            "reserved_gib_start": mem0.get("reserved_bytes.all.current", 0) / 2 ** 30,
+            "reserved_gib_peak": mem1.get("reserved_bytes.all.peak", 0) / 2 ** 30,
+            "host_ops": {k: {"calls": n, "total_s": round(t, 4), "max_s": round(m, 4)}
+                         for k, (n, t, m) in host_ops.items()},
             "plain_grid_gb": [round(min(plain_gb), 3), round(max(plain_gb), 3)],
             "makespan_s": makespan, "pose_steps": ps, "aggregate_pose_steps_per_s": ps / makespan,
             "poses_per_job": [int(min(nposes)), int(max(nposes))], "chunk_steps": sch.chunk_steps}
         log(f"[bench] serving: p50 {result['serving']['p50_latency_s']:.3f}s p99 "
             f"{result['serving']['p99_latency_s']:.3f}s, admission wait p99 "
             f"{result['serving']['p99_admission_wait_s']:.3f}s, {sch.evictions} evictions, {corner} corner / "
-            f"{len(be.layouts) - corner} plain grids, {ps / makespan:.3e} pose-steps/s over {makespan:.1f}s")
+            f"{len(be.layouts) - corner} plain grids, {ps / makespan:.3e} pose-steps/s over {makespan:.1f}s; "
+            f"longest tick {np.max(ticks):.3f}s, allocator retries {result['serving']['allocator_retries']}, "
+            f"reserved {result['serving']['reserved_gib_start']:.1f} -> peak "
+            f"{result['serving']['reserved_gib_peak']:.1f} GiB")
 
     print(json.dumps(result), flush=True)
 
