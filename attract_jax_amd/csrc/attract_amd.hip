@@ -383,7 +383,8 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, r2, kl) and 8-byte side record (kh, gh) of one pair type.
+// 16-byte pair record (rc, ac, r2, q) and 8-byte side record (kl, kh) of one pair type, q being
+// the force-shift factor gh with the plateau and the approximate derivative, else kl.
 // ATTRACT's pair energy (test-grid-energrad.py:95-111) is vlj + (ivor - 1) emin for
 // dsq < rmin2 and ivor vlj beyond, ivor = +-1 (ipon, attract-original/
 // convert-attract-par.py:14-15); an ivor of another magnitude poisons the record (NaN
@@ -396,7 +397,8 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
 // s = lo ? 1 : -1 either way.  The gradient factor is -2 (s dh + gh / sqrt(dsq)) (see
 // pair_eval), gh = P dE/ddsq (P) being the force-shift factor -2 P dE/ddsq (P)
 // (test-grid-energrad.py:124-147) divided by -2; zero for the exact derivative and without
-// the plateau.  Without the plateau kh = 0 and the side record is not read.
+// the plateau.  The gradient needs the 16-byte record only; the side record holds the energy
+// constants (kh = 0 without the plateau, where it is not read).
 struct PairType {
   float4 rec;
   float2 side;
@@ -408,15 +410,14 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
   const float w = (iv - 1.f) * em;
   PairType t;
   t.rec = make_float4(rc, ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, w);
-  t.side = make_float2(0.f, 0.f);
+  t.side = make_float2(w, 0.f);
   if (plateau) {
     const float vljP = fmaf(rc, pc.uP4, -(ac * pc.uP3));
     const bool loP = pc.P2 < r2;
     const float sP = loP ? 1.f : iv;
     const float EP = fmaf(sP, vljP, loP ? w : 0.f);
-    t.rec.w = w - EP;
-    t.side.x = -EP;
-    if (!exact) t.side.y = 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
+    t.side = make_float2(w - EP, -EP);
+    t.rec.w = exact ? w - EP : 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
   }
   return t;
 }
@@ -426,8 +427,9 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
 // carries them).  Returns E and h such that dE/dx'_lig = -2 h (x'_lig - x_rec):
 // h = -s d vlj / d dsq (+ gh / sqrt(dsq)); the factor -2 is applied by the caller once per
 // list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
-// pairs beyond the plateau).  Without the plateau the side record is not read (kh = 0).
-template <bool PLATEAU, bool EXACT>
+// pairs beyond the plateau).  Without the plateau the side record is not read (kh = 0);
+// without ENERGY it is not read at all and E is not formed.
+template <bool PLATEAU, bool EXACT, bool ENERGY = true>
 __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& h) {
   float u, rs = 0.f;
   if (PLATEAU && !EXACT) {
@@ -442,10 +444,11 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
   const bool lo = dsq < pa.z;  // always for ivor = +1 (r2 = FLT_MAX)
   const float u3 = u2 * u;
   const float su3 = lo ? u3 : -u3;  // s u^3
-  E = fmaf(t1, su3, lo ? pa.w : (PLATEAU ? ps.x : 0.f));
+  // q = kl except with the plateau and the approximate derivative, where the side record has it
+  if (ENERGY) E = fmaf(t1, su3, lo ? (PLATEAU && !EXACT ? ps.x : pa.w) : (PLATEAU ? ps.y : 0.f));
   // s d vlj / d dsq = -(4 rc u - 3 ac) s u^4 = -(3 t1 + rep) s u^4
   h = fmaf(3.f, t1, rep) * (su3 * u);
-  if (PLATEAU && !EXACT) h = fmaf(ps.y, rs, h);
+  if (PLATEAU && !EXACT) h = fmaf(pa.w, rs, h);
 }
 
 // pair_eval with run-time flags, returning the gradient factor gs = -2 h:
@@ -578,8 +581,9 @@ __device__ __forceinline__ float4 clamp_rec_type(float4 b, int nrt) {
 // Pair loop over the flattened list segments (S entries each) of one chunk of beads.
 // Bead lane j brings its segment count `segs` and first item `excl`; each window of 64
 // items finds its beads by marking the segment starts that fall in the window and taking
-// a prefix maximum (the bead of the window's first item is carried over).
-template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT>
+// a prefix maximum (the bead of the window's first item is carried over).  Without ENERGY
+// (minimizer steps whose energy is not kept) the pair energies are not formed.
+template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, bool ENERGY>
 __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
                                            const float4* __restrict__ s_par, const float2* __restrict__ s_side,
                                            float* s_pgrad, WaveScratch* ws, int total, int segs, int excl,
@@ -654,7 +658,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         const unsigned pi = ok[cc] ? __umul24(rt, stride) + blt : nslots;
         pidx[cc] = pi;
         pa[cc] = s_par[pi];
-        ps[cc] = PLATEAU ? s_side[pi] : make_float2(0.f, 0.f);
+        ps[cc] = PLATEAU && ENERGY && MODE != kAdjoint ? s_side[pi] : make_float2(0.f, 0.f);
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
@@ -679,9 +683,9 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
             if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
           }
         } else {
-          float E, h;
-          pair_eval<PLATEAU, EXACT>(dsq[cc], pa[cc], ps[cc], E, h);
-          e += E;
+          float E = 0.f, h;
+          pair_eval<PLATEAU, EXACT, ENERGY>(dsq[cc], pa[cc], ps[cc], E, h);
+          if (ENERGY) e += E;
           gx = fmaf(h, dd[cc][0], gx);
           gy = fmaf(h, dd[cc][1], gy);
           gz = fmaf(h, dd[cc][2], gz);
@@ -691,7 +695,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
     if (MODE == kAdjoint) continue;
     // the gradient is -2 (gx, gy, gz); bl holds -2 x_local, and scaling by -2 is exact, so
     // fma(-2 x, g', acc) and fma(-2, g', acc) equal the products with the scaled gradient
-    acc[0] += e;
+    if (ENERGY) acc[0] += e;
     const float g[3] = {gx, gy, gz};
     const float xb[3] = {bl.x, bl.y, bl.z};
 #pragma unroll
@@ -740,11 +744,13 @@ struct PoseIn {
 // beads run on lanes 32 + j in the same bead phase, and its grid sums wait in LDS while A's
 // pairs are evaluated, then move to lanes 0-31.  Either way a pose's sums are formed on the
 // same lanes in the same order (grid terms of bead j on lane j, then its pairs in windows of
-// its own, one reduction), so its result is the same alone or beside another.
+// its own, one reduction), so its result is the same alone or beside another.  With energy
+// false (minimizer steps whose energy is not kept) the pair energies are left out of the
+// energy sum; every gradient sum is the same.
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws, const PoseIn& A,
-                              const PoseIn& B, WaveStats& st, int lane, float& uA, float& uB) {
+                              const PoseIn& B, WaveStats& st, int lane, float& uA, float& uB, bool energy = true) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
@@ -860,16 +866,28 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       }
       const int n8 = u8 ? (tail4 ? nfull : tot) : 0;
       const int n4 = u8 ? (tail4 ? tot4t : 0) : tot;
-      if (n8 > 0)
-        pair_items<8, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt, acc,
-                                                     npair, lane);
+      const bool no_energy = MODE == kMinimize && !energy;
+      if (n8 > 0) {
+        if (no_energy)
+          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt,
+                                                              acc, npair, lane);
+        else
+          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt,
+                                                             acc, npair, lane);
+      }
       if (tail4) {
         if (h == q) ws->xl[lane].w = __int_as_float(lstart + done - 4 * excl4t);
         wave_sync();
       }
-      if (n4 > 0)
-        pair_items<4, MODE, REC_LDS, PLATEAU, EXACT>(a, recp, s_par, s_side, s_pgrad, ws, n4, tail4 ? seg4t : sq,
-                                                     tail4 ? excl4t : excl, wgt, acc, npair, lane);
+      if (n4 > 0) {
+        const int sq4 = tail4 ? seg4t : sq, ex4 = tail4 ? excl4t : excl;
+        if (no_energy)
+          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_pgrad, ws, n4, sq4, ex4, wgt,
+                                                              acc, npair, lane);
+        else
+          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_pgrad, ws, n4, sq4, ex4, wgt,
+                                                             acc, npair, lane);
+      }
     }
     phase_end(st, kPhPair);
     st.n[3] += lane == 0 ? npair : 0u;
@@ -1166,9 +1184,12 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
             P[1].M = bs->M[c];
           }
         }
+        // the energy is kept only from a slot's last step of the launch
+        bool energy = s + 1 >= __builtin_amdgcn_readlane(nst, b);
+        if (partner >= 0) energy = energy || s + 1 >= __builtin_amdgcn_readlane(nst, partner);
         float uA, uB;
         pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws, P[0], P[1],
-                                                                  st, lane, uA, uB);
+                                                                  st, lane, uA, uB, energy);
         const int i = acc_index(lane);
         if (!(lane & 3) && i < kAcc) {
           bs->acc[b][i] = uA;
