@@ -117,6 +117,16 @@ def ATJ_LAUNCH_CLAIM(n):
     return int(n) << 4
 
 
+ATJ_LAUNCH_TAIL_MASK = 0xF00
+ATJ_LAUNCH_TAIL_ROUNDS_MASK = 0xF000
+
+
+def ATJ_LAUNCH_TAIL(n, rounds=1):
+    """Launch option: the poses of the last `rounds` rounds of claims (1..15) are claimed n at a
+    time (1, 2 or 4, below the claim size)."""
+    return (int(n) << 8) | (int(rounds) << 12)
+
+
 class AtjLaunch(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks", ctypes.c_int32), ("options", ctypes.c_uint32)]
 

@@ -1,8 +1,8 @@
 """Launch-shape autotuner of the persistent pose kernels.
 
 Times a short persistent minimization for each candidate -- grid layout (corner-packed or
-plain), waves per block, receptor in LDS or read through L1, poses per work claim -- and keeps
-the fastest.  This replaces the reference's hand-set batch knobs (test3.py:7-12 BATCH /
+plain), waves per block, receptor in LDS or read through L1, poses per work claim and a tail of
+smaller claims -- and keeps the fastest.  This replaces the reference's hand-set batch knobs (test3.py:7-12 BATCH /
 NB_CHUNK, test4.py:99-103 nb_chunk_thresholds), which sized XLA chunks for the CPU.  Results
 are cached per process by (receptor size, grid dims, ligand-size histogram bucket,
 candidate set)."""
@@ -16,7 +16,10 @@ from .minimize import BatchPlan, PoseState, minimize
 
 _L1 = _lib.ATJ_LAUNCH_REC_GLOBAL
 SHAPES = ((4, 0), (8, 0), (16, 0), (16, _L1), (8, _L1))
-CLAIMS = (0, 4, 2)  # 0: the launch's own choice (8 poses per claim for large launches)
+# claim options: the launch's own choice (8 poses per claim for large launches), claims of 4 or
+# 2, and claims of 8 whose last one or two rounds are claimed 4 at a time
+CLAIMS = (0, _lib.ATJ_LAUNCH_CLAIM(4), _lib.ATJ_LAUNCH_CLAIM(2), _lib.ATJ_LAUNCH_CLAIM(8) | _lib.ATJ_LAUNCH_TAIL(4, 1),
+          _lib.ATJ_LAUNCH_CLAIM(8) | _lib.ATJ_LAUNCH_TAIL(4, 2))
 _cache = {}
 
 
@@ -43,7 +46,7 @@ def autotune_minimize(systems, plan, w, device, shapes=SHAPES, claims=CLAIMS, st
         old = system.launch
         for wpb, opt in shapes:
             for claim in claims:
-                o = opt | (_lib.ATJ_LAUNCH_CLAIM(claim) if claim else 0)
+                o = opt | claim
                 system.launch = _lib.AtjLaunch(wpb, 0, o)
                 times = []
                 for rep in range(3):

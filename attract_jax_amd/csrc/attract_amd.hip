@@ -383,25 +383,26 @@ __device__ __forceinline__ PlateauConst plateau_const(float P2) {
   return c;
 }
 
-// 16-byte pair record (rc, ac, r2, q) and 8-byte side record (kl, kh) of one pair type, q being
-// the force-shift factor gh with the plateau and the approximate derivative, else kl.
-// ATTRACT's pair energy (test-grid-energrad.py:95-111) is vlj + (ivor - 1) emin for
-// dsq < rmin2 and ivor vlj beyond, ivor = +-1 (ipon, attract-original/
-// convert-attract-par.py:14-15); an ivor of another magnitude poisons the record (NaN
-// energies) rather than being misread.  With
-// w = (ivor - 1) emin, lo = dsq < rmin2 and s = lo ? 1 : ivor, both branches are s vlj + k
-// with k = kl = w on the low side and k = kh = 0 beyond (each minus E(plateaudis) with the
-// plateau); vlj = t1 u^3, so E = t1 (s u^3) + k, and s folds into u^3 by one select.  For
+// Pair-type records: a 16-byte record (rc, -3/4 ac, r2, gh / 4), an 8-byte side record
+// (ac, kl) and a 4-byte kh.  ATTRACT's pair energy (test-grid-energrad.py:95-111) is
+// vlj + (ivor - 1) emin for dsq < rmin2 and ivor vlj beyond, ivor = +-1 (ipon,
+// attract-original/convert-attract-par.py:14-15); an ivor of another magnitude poisons the
+// record (NaN energies) rather than being misread.  With w = (ivor - 1) emin,
+// lo = dsq < rmin2 and s = lo ? 1 : ivor, both branches are s vlj + k with k = kl = w on the
+// low side and k = kh = 0 beyond (each minus E(plateaudis) with the plateau); vlj = t1 u^3
+// with t1 = rc u - ac, so E = t1 (s u^3) + k, and s folds into u^3 by one select.  For
 // ivor = +1, w = 0 and s = 1 on both sides of rmin2, so the record stores r2 = FLT_MAX and
 // lo always holds; for ivor = -1 it stores r2 = rmin2 (infinity for rmin2 >= FLT_MAX), and
-// s = lo ? 1 : -1 either way.  The gradient factor is -2 (s dh + gh / sqrt(dsq)) (see
-// pair_eval), gh = P dE/ddsq (P) being the force-shift factor -2 P dE/ddsq (P)
-// (test-grid-energrad.py:124-147) divided by -2; zero for the exact derivative and without
-// the plateau.  The gradient needs the 16-byte record only; the side record holds the energy
-// constants (kh = 0 without the plateau, where it is not read).
+// s = lo ? 1 : -1 either way.  The gradient needs the 16-byte record only:
+// -s d vlj / d dsq = (4 rc u - 3 ac) s u^4 = 4 (rc u - 3/4 ac) s u^4, and the force shift
+// adds gh / sqrt(dsq), gh = P dE/ddsq (P) being the force-shift factor -2 P dE/ddsq (P)
+// (test-grid-energrad.py:124-147) divided by -2 (zero for the exact derivative and without
+// the plateau); the record carries both terms divided by 4.  The side record and kh hold the
+// energy constants (kh = 0 without the plateau, where it is not read).
 struct PairType {
   float4 rec;
   float2 side;
+  float kh;
 };
 
 __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, float em, float r2, bool plateau,
@@ -409,28 +410,31 @@ __device__ __forceinline__ PairType pair_type(float rc, float ac, float iv, floa
   if (fabsf(iv) != 1.f) rc = __int_as_float(0x7fc00000);
   const float w = (iv - 1.f) * em;
   PairType t;
-  t.rec = make_float4(rc, ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, w);
-  t.side = make_float2(w, 0.f);
+  t.rec = make_float4(rc, -0.75f * ac, iv < 0.f ? (r2 >= FLT_MAX ? INFINITY : r2) : FLT_MAX, 0.f);
+  t.side = make_float2(ac, w);
+  t.kh = 0.f;
   if (plateau) {
     const float vljP = fmaf(rc, pc.uP4, -(ac * pc.uP3));
     const bool loP = pc.P2 < r2;
     const float sP = loP ? 1.f : iv;
     const float EP = fmaf(sP, vljP, loP ? w : 0.f);
-    t.side = make_float2(w - EP, -EP);
-    t.rec.w = exact ? w - EP : 0.5f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
+    t.side.y = w - EP;
+    t.kh = -EP;
+    if (!exact) t.rec.w = 0.125f * (sP * fmaf(rc, pc.k1, ac * pc.k2));
   }
   return t;
 }
 
 // ATTRACT 8-6 pair potential (test-grid-energrad.py:95-111) and its dsq-derivative, with
-// the pair-type terms of the record (plateau subtraction and force shift when the record
-// carries them).  Returns E and h such that dE/dx'_lig = -2 h (x'_lig - x_rec):
-// h = -s d vlj / d dsq (+ gh / sqrt(dsq)); the factor -2 is applied by the caller once per
-// list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
-// pairs beyond the plateau).  Without the plateau the side record is not read (kh = 0);
-// without ENERGY it is not read at all and E is not formed.
+// the pair-type terms of the records (plateau subtraction and force shift when they carry
+// them).  Returns E and h such that dE/dx'_lig = -8 h (x'_lig - x_rec):
+// h = -s d vlj / d dsq / 4 (+ gh / (4 sqrt(dsq))); the factor -8 is applied by the caller once
+// per list segment.  The all-zero record gives E = h = 0 for any finite dsq > 0 (padding and
+// pairs beyond the plateau).  Without ENERGY only the 16-byte record is read and E is not
+// formed; without the plateau kh is not read.
 template <bool PLATEAU, bool EXACT, bool ENERGY = true>
-__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float& E, float& h) {
+__device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const float2 ps, float kh, float& E,
+                                          float& h) {
   float u, rs = 0.f;
   if (PLATEAU && !EXACT) {
     rs = __builtin_amdgcn_rsqf(dsq);
@@ -439,32 +443,32 @@ __device__ __forceinline__ void pair_eval(float dsq, const float4 pa, const floa
     u = __builtin_amdgcn_rcpf(dsq);
   }
   const float u2 = u * u;
-  const float rep = pa.x * u;
-  const float t1 = rep - pa.y;
   const bool lo = dsq < pa.z;  // always for ivor = +1 (r2 = FLT_MAX)
-  const float u3 = u2 * u;
-  const float su3 = lo ? u3 : -u3;  // s u^3
-  // q = kl except with the plateau and the approximate derivative, where the side record has it
-  if (ENERGY) E = fmaf(t1, su3, lo ? (PLATEAU && !EXACT ? ps.x : pa.w) : (PLATEAU ? ps.y : 0.f));
-  // s d vlj / d dsq = -(4 rc u - 3 ac) s u^4 = -(3 t1 + rep) s u^4
-  h = fmaf(3.f, t1, rep) * (su3 * u);
+  const float u4 = u2 * u2;
+  const float su4 = lo ? u4 : -u4;  // s u^4
+  h = fmaf(pa.x, u, pa.y) * su4;    // (rc u - 3/4 ac) s u^4
   if (PLATEAU && !EXACT) h = fmaf(pa.w, rs, h);
+  if (ENERGY) {
+    const float t1 = fmaf(pa.x, u, -ps.x);  // rc u - ac
+    const float u3 = u2 * u;
+    E = fmaf(t1, lo ? u3 : -u3, lo ? ps.y : (PLATEAU ? kh : 0.f));
+  }
 }
 
-// pair_eval with run-time flags, returning the gradient factor gs = -2 h:
+// pair_eval with run-time flags, returning the gradient factor gs = -8 h:
 // dE/dx'_lig = gs (x'_lig - x_rec).
 __device__ __forceinline__ void pair_eval_rt(bool plateau, bool exact, float dsq, const PairType& t, float& E,
                                              float& gs) {
   float h;
   if (plateau) {
     if (exact)
-      pair_eval<true, true>(dsq, t.rec, t.side, E, h);
+      pair_eval<true, true>(dsq, t.rec, t.side, t.kh, E, h);
     else
-      pair_eval<true, false>(dsq, t.rec, t.side, E, h);
+      pair_eval<true, false>(dsq, t.rec, t.side, t.kh, E, h);
   } else {
-    pair_eval<false, false>(dsq, t.rec, t.side, E, h);
+    pair_eval<false, false>(dsq, t.rec, t.side, t.kh, E, h);
   }
-  gs = -2.f * h;
+  gs = -8.f * h;
 }
 
 // Partial derivatives of nonbon(dsq) w.r.t. (rc, ac, emin) as independent inputs; u = 1/dsq.
@@ -484,12 +488,18 @@ __device__ __forceinline__ void pair_param_grad(float dsq, float u, const float4
 // all-zero record.
 __host__ __device__ __forceinline__ int pair_stride(int nlt) { return nlt | 1; }
 __host__ __device__ __forceinline__ int pair_slots(int nrt, int nlt) { return nrt * pair_stride(nlt); }
-// Side-record slots: nslots + 1 rounded up to even, keeping what follows 16-byte aligned.
+// Side-record slots: nslots + 1 rounded up to even, and kh slots: nslots + 1 rounded up to a
+// multiple of 4, keeping what follows 16-byte aligned.
 __host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots + 2) & ~1; }
+__host__ __device__ __forceinline__ int kh_slots(int nslots) { return (nslots + 4) & ~3; }
+// LDS bytes of the staged pair-type records.
+__host__ __device__ __forceinline__ size_t pair_lds_bytes(int nslots) {
+  return (size_t)(nslots + 1) * 16 + (size_t)side_slots(nslots) * 8 + (size_t)kh_slots(nslots) * 4;
+}
 
 struct WaveScratch {
-  float4 xp[kWave];          // x', y', z', ligand ff type (int bits)
-  float4 xl[kWave];          // -2 (x, y, z) (local), list start - S * first item (int bits)
+  float4 xp[kWave];          // x', y', z', 16 x ligand ff type (the pair-record column in bytes, int bits)
+  float4 xl[kWave];          // -8 (x, y, z) (local), list start - S * first item (int bits)
   signed char mark[kWave];   // bead whose first list segment is item base + i of the current window
   float4 gridB[kHalf];       // grid energy and gradient of the beads of the second pose of a shared bead phase
 };
@@ -537,6 +547,8 @@ struct KArgs {
   int nsteps_launch;
   unsigned flags;
   int batch;          // poses per work-counter claim, 1..kBatch
+  int tail_batch;     // poses per claim from tail_start on (<= batch)
+  int tail_start;     // first pose of the tail (a multiple of batch; N: no tail)
   int static_blocks;  // blocks whose wavefronts take their first claim by index (the resident ones)
   int* counter;
   unsigned long long* stats;
@@ -582,18 +594,22 @@ __device__ __forceinline__ float4 clamp_rec_type(float4 b, int nrt) {
 // Bead lane j brings its segment count `segs` and first item `excl`; each window of 64
 // items finds its beads by marking the segment starts that fall in the window and taking
 // a prefix maximum (the bead of the window's first item is carried over).  Without ENERGY
-// (minimizer steps whose energy is not kept) the pair energies are not formed.
+// (minimizer steps whose energy is not kept) the pair energies are not formed.  A pair's
+// record is addressed in bytes: the receptor bead's row (its .w in LDS; type x row bytes for
+// beads read through L1) plus the ligand bead's column (ws->xp .w).
 template <int S, int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, bool ENERGY>
 __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restrict__ recp,
                                            const float4* __restrict__ s_par, const float2* __restrict__ s_side,
-                                           float* s_pgrad, WaveScratch* ws, int total, int segs, int excl,
-                                           float wgt, float acc[kAcc], unsigned& npair, int lane) {
+                                           const float* __restrict__ s_kh, float* s_pgrad, WaveScratch* ws,
+                                           int total, int segs, int excl, float wgt, float acc[kAcc],
+                                           unsigned& npair, int lane) {
   const GridView& G = a.G;
   const float P2 = G.p2;
   const float uP2 = 1.f / P2;  // (adjoint) the plateau terms' 1/dsq
   const unsigned nrec = (unsigned)a.nrec;
-  const unsigned stride = (unsigned)pair_stride(a.nlt);
+  const unsigned row_bytes = 16u * (unsigned)pair_stride(a.nlt);
   const unsigned nslots = (unsigned)pair_slots(a.nrt, a.nlt);
+  const unsigned zero_rec = 16u * nslots;  // byte offset of the all-zero record
   int carry = 0;
   for (int base = 0; base < total; base += kWave) {
     ws->mark[lane] = -1;
@@ -607,7 +623,8 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
     const bool act = item < total;
     const float4 bx = ws->xp[jb];
     const float4 bl = ws->xl[jb];
-    const unsigned blt = (unsigned)__float_as_int(bx.w);
+    const unsigned col = (unsigned)__float_as_int(bx.w);  // 16 x ligand ff type
+    const unsigned zrow = zero_rec - col;                 // the "row" whose column is the zero record
     // inactive lanes read the all-padding segment at the head of the list array
     const int off = act ? __float_as_int(bl.w) + item * S : 0;
     static_assert(S == 4 || S == 8, "segments of 4 or 8 list entries");
@@ -642,6 +659,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       float dd[4][3], dsq[4];
       float4 pa[4];
       float2 ps[4];
+      float kh[4];
       bool ok[4];
       unsigned pidx[4];
 #pragma unroll
@@ -651,14 +669,15 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         dd[cc][2] = bx.z - rb[cc].z;
         dsq[cc] = fmaf(dd[cc][2], dd[cc][2], fmaf(dd[cc][1], dd[cc][1], dd[cc][0] * dd[cc][0]));
         ok[cc] = dsq[cc] < P2;
-        // the receptor type is read whatever ok is: an empty asm keeps the compiler from
+        // the receptor's row is read whatever ok is: an empty asm keeps the compiler from
         // turning the select below into a branch around that read
-        unsigned rt = (unsigned)__float_as_int(rb[cc].w);
-        asm("" : "+v"(rt));
-        const unsigned pi = ok[cc] ? __umul24(rt, stride) + blt : nslots;
-        pidx[cc] = pi;
-        pa[cc] = s_par[pi];
-        ps[cc] = PLATEAU && ENERGY && MODE != kAdjoint ? s_side[pi] : make_float2(0.f, 0.f);
+        unsigned row = (unsigned)__float_as_int(rb[cc].w);
+        asm("" : "+v"(row));
+        const unsigned pb = (ok[cc] ? (REC_LDS ? row : __umul24(row, row_bytes)) : zrow) + col;
+        pidx[cc] = pb;
+        pa[cc] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_par) + pb);
+        ps[cc] = ENERGY && MODE != kAdjoint ? s_side[pb >> 4] : make_float2(0.f, 0.f);
+        kh[cc] = PLATEAU && ENERGY && MODE != kAdjoint ? s_kh[pb >> 4] : 0.f;
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
@@ -676,7 +695,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
               dp[1] -= dq[1];
               dp[2] -= dq[2];
             }
-            const unsigned pg = pidx[cc];
+            const unsigned pg = pidx[cc] >> 4;
             atomicAdd(&s_pgrad[pg], wgt * dp[0]);
             atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
             // the emin derivative is zero except for repulsive pairs inside rmin
@@ -684,7 +703,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
           }
         } else {
           float E = 0.f, h;
-          pair_eval<PLATEAU, EXACT, ENERGY>(dsq[cc], pa[cc], ps[cc], E, h);
+          pair_eval<PLATEAU, EXACT, ENERGY>(dsq[cc], pa[cc], ps[cc], kh[cc], E, h);
           if (ENERGY) e += E;
           gx = fmaf(h, dd[cc][0], gx);
           gy = fmaf(h, dd[cc][1], gy);
@@ -693,8 +712,8 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
       }
     }
     if (MODE == kAdjoint) continue;
-    // the gradient is -2 (gx, gy, gz); bl holds -2 x_local, and scaling by -2 is exact, so
-    // fma(-2 x, g', acc) and fma(-2, g', acc) equal the products with the scaled gradient
+    // the gradient is -8 (gx, gy, gz); bl holds -8 x_local, and scaling by -8 is exact, so
+    // fma(-8 x, g', acc) and fma(-8, g', acc) equal the products with the scaled gradient
     if (ENERGY) acc[0] += e;
     const float g[3] = {gx, gy, gz};
     const float xb[3] = {bl.x, bl.y, bl.z};
@@ -703,7 +722,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
 #pragma unroll
       for (int l = 0; l < 3; ++l) acc[1 + 3 * k + l] = fmaf(xb[k], g[l], acc[1 + 3 * k + l]);
 #pragma unroll
-    for (int l = 0; l < 3; ++l) acc[10 + l] = fmaf(-2.f, g[l], acc[10 + l]);
+    for (int l = 0; l < 3; ++l) acc[10 + l] = fmaf(-8.f, g[l], acc[10 + l]);
   }
 }
 
@@ -749,8 +768,9 @@ struct PoseIn {
 // energy sum; every gradient sum is the same.
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              const float2* __restrict__ s_side, float* s_pgrad, WaveScratch* ws, const PoseIn& A,
-                              const PoseIn& B, WaveStats& st, int lane, float& uA, float& uB, bool energy = true) {
+                              const float2* __restrict__ s_side, const float* __restrict__ s_kh, float* s_pgrad,
+                              WaveScratch* ws, const PoseIn& A, const PoseIn& B, WaveStats& st, int lane, float& uA,
+                              float& uB, bool energy = true) {
   const float4* __restrict__ recp = REC_LDS ? s_rec : a.rec;
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
@@ -829,8 +849,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     const int lstart = start;  // the bead's list start
     start -= excl * (h ? (u8B ? 8 : 4) : (u8A ? 8 : 4));
     if (nposes == 2 || t8A + t8B > 0) {
-      ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(lt));
-      ws->xl[lane] = make_float4(-2.f * xl[0], -2.f * xl[1], -2.f * xl[2], __int_as_float(start));
+      ws->xp[lane] = make_float4(xp[0], xp[1], xp[2], __int_as_float(16 * lt));
+      ws->xl[lane] = make_float4(-8.f * xl[0], -8.f * xl[1], -8.f * xl[2], __int_as_float(start));
     }
     wave_sync();
     phase_end(st, kPhScan);
@@ -842,8 +862,8 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
         if (!h) {
           const float4 e = ws->gridB[lane];
-          const float4 x = ws->xl[kHalf + lane];  // -2 x_local
-          const float gb[3] = {e.y, e.z, e.w}, xb[3] = {-0.5f * x.x, -0.5f * x.y, -0.5f * x.z};
+          const float4 x = ws->xl[kHalf + lane];  // -8 x_local
+          const float gb[3] = {e.y, e.z, e.w}, xb[3] = {-0.125f * x.x, -0.125f * x.y, -0.125f * x.z};
           bead_acc(acc, e.x, gb, xb);
         }
       }
@@ -869,10 +889,10 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const bool no_energy = MODE == kMinimize && !energy;
       if (n8 > 0) {
         if (no_energy)
-          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt,
+          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_kh, s_pgrad, ws, n8, sq, excl, wgt,
                                                               acc, npair, lane);
         else
-          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_pgrad, ws, n8, sq, excl, wgt,
+          pair_items<8, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_kh, s_pgrad, ws, n8, sq, excl, wgt,
                                                              acc, npair, lane);
       }
       if (tail4) {
@@ -882,10 +902,10 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       if (n4 > 0) {
         const int sq4 = tail4 ? seg4t : sq, ex4 = tail4 ? excl4t : excl;
         if (no_energy)
-          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_pgrad, ws, n4, sq4, ex4, wgt,
+          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, false>(a, recp, s_par, s_side, s_kh, s_pgrad, ws, n4, sq4, ex4, wgt,
                                                               acc, npair, lane);
         else
-          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_pgrad, ws, n4, sq4, ex4, wgt,
+          pair_items<4, MODE, REC_LDS, PLATEAU, EXACT, true>(a, recp, s_par, s_side, s_kh, s_pgrad, ws, n4, sq4, ex4, wgt,
                                                              acc, npair, lane);
       }
     }
@@ -1061,52 +1081,77 @@ __device__ __forceinline__ void load_pose_matrix(const float* p, bool qt, float 
 
 
 // Stage the receptor beads (+ the padding sentinel) and the pair table (+ the
-// all-zero record) into LDS.
-__device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* s_side, bool plateau, bool exact,
-                          float* s_pgrad, int npgrad) {
+// all-zero record) into LDS.  A staged bead's .w is the byte offset of its pair-record row.
+__device__ void stage_lds(const KArgs& a, float4* s_rec, float4* s_par, float2* s_side, float* s_kh, bool plateau,
+                          bool exact, float* s_pgrad, int npgrad) {
   const int tid = threadIdx.x;
+  const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
   if (s_rec) {
-    for (int i = tid; i < a.nrec; i += blockDim.x) s_rec[i] = clamp_rec_type(a.rec[i], a.nrt);
+    for (int i = tid; i < a.nrec; i += blockDim.x) {
+      float4 b = clamp_rec_type(a.rec[i], a.nrt);
+      b.w = __int_as_float(16 * stride * __float_as_int(b.w));
+      s_rec[i] = b;
+    }
     if (tid == 0) s_rec[a.nrec] = sentinel_bead();
   }
-  const int stride = pair_stride(a.nlt), nslots = pair_slots(a.nrt, a.nlt);
   const PlateauConst pc = plateau_const(a.G.p2);
   for (int i = tid; i <= nslots; i += blockDim.x) {
     const int rt = i / stride, lt = i - rt * stride;
     PairType t;
     t.rec = make_float4(0.f, 0.f, 0.f, 0.f);
     t.side = make_float2(0.f, 0.f);
+    t.kh = 0.f;
     if (i < nslots && lt < a.nlt) {
       const int k = rt * a.nlt + lt;
       t = pair_type(a.rc[k], a.ac[k], a.ivor[k], a.emin[k], a.rmin2[k], plateau, exact, pc);
     }
     s_par[i] = t.rec;
     s_side[i] = t.side;
+    s_kh[i] = t.kh;
   }
   if (s_pgrad)
     for (int i = tid; i < npgrad; i += blockDim.x) s_pgrad[i] = 0.f;
 }
 
-// Work claims of a persistent wavefront, `a.batch` poses each.  In the blocks the device holds
-// at once (a.static_blocks) a wavefront's first claim is fixed by its index; every other claim
+// Work claims of a persistent wavefront.  Claim k covers a.batch poses from k a.batch on, up
+// to a.tail_start, and a.tail_batch poses each after it.  In the blocks the device holds at
+// once (a.static_blocks) a wavefront's first claim is fixed by its index; every other claim
 // comes from the work counter and follows those first claims.  A launch whose first claims
 // cover every pose thus takes no atomics (one per wavefront on a single address made small
-// launches atomic-bound), and a larger one takes none for its first round.  Returns the
-// claim's first pose (>= N: no work left).
-__device__ __forceinline__ long long first_round(const KArgs& a) {
-  return (long long)a.static_blocks * (blockDim.x / kWave) * a.batch;
+// launches atomic-bound), and a larger one takes none for its first round.
+struct Claim {
+  long long k0;  // first pose of the claim (>= N: no work left)
+  int n;         // poses in it
+};
+__device__ __forceinline__ long long claim_count(const KArgs& a) {
+  const long long kb = a.tail_start / a.batch;
+  return kb + (a.N - a.tail_start + a.tail_batch - 1) / a.tail_batch;
 }
-__device__ __forceinline__ long long next_claim(const KArgs& a, int lane) {
-  const long long fr = first_round(a);
-  if (fr >= a.N) return a.N;
+__device__ __forceinline__ Claim claim_of(const KArgs& a, long long k) {
+  const long long kb = a.tail_start / a.batch;
+  Claim c;
+  if (k < kb) {
+    c.k0 = k * a.batch;
+    c.n = a.batch;
+  } else {
+    c.k0 = a.tail_start + (k - kb) * a.tail_batch;
+    c.n = a.tail_batch;
+  }
+  if (c.k0 >= a.N) c.k0 = a.N;
+  c.n = (int)min((long long)c.n, a.N - c.k0);
+  return c;
+}
+__device__ __forceinline__ Claim next_claim(const KArgs& a, int lane) {
+  const long long fr = (long long)a.static_blocks * (blockDim.x / kWave);
+  if (fr >= claim_count(a)) return claim_of(a, claim_count(a));
   int c = 0;
-  if (lane == 0) c = atomicAdd(a.counter, a.batch);
-  return fr + __builtin_amdgcn_readfirstlane(c);
+  if (lane == 0) c = atomicAdd(a.counter, 1);
+  return claim_of(a, fr + __builtin_amdgcn_readfirstlane(c));
 }
-__device__ __forceinline__ long long first_claim(const KArgs& a, int lane) {
+__device__ __forceinline__ Claim first_claim(const KArgs& a, int lane) {
   if ((int)blockIdx.x >= a.static_blocks) return next_claim(a, lane);
   const int wave = uni((int)(threadIdx.x / kWave));  // uniform across the wavefront
-  return ((long long)blockIdx.x * (blockDim.x / kWave) + wave) * a.batch;
+  return claim_of(a, (long long)blockIdx.x * (blockDim.x / kWave) + wave);
 }
 
 // Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
@@ -1116,13 +1161,13 @@ __device__ __forceinline__ long long first_claim(const KArgs& a, int lane) {
 // different jobs and have different step counts.
 template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
-                              const float2* __restrict__ s_side, WaveScratch* ws, BatchScratch* bs, WaveStats& st,
-                              int lane) {
+                              const float2* __restrict__ s_side, const float* __restrict__ s_kh, WaveScratch* ws,
+                              BatchScratch* bs, WaveStats& st, int lane) {
   // Slot lanes of the optimizer update: lane 8 b + i updates parameter i (< 7) of slot b.
   const int sb = lane >> 3, pi = lane & 7;
-  for (long long claim = first_claim(a, lane); claim < a.N; claim = next_claim(a, lane)) {
-    const int k0 = (int)claim;
-    const int nbatch = min(a.batch, a.N - k0);
+  for (Claim claim = first_claim(a, lane); claim.k0 < a.N; claim = next_claim(a, lane)) {
+    const int k0 = (int)claim.k0;
+    const int nbatch = claim.n;
     const bool own = lane < nbatch;
     int p = 0, jid = 0, cnt = 0, nst = 0;
     if (own) {
@@ -1188,8 +1233,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
         bool energy = s + 1 >= __builtin_amdgcn_readlane(nst, b);
         if (partner >= 0) energy = energy || s + 1 >= __builtin_amdgcn_readlane(nst, partner);
         float uA, uB;
-        pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, nullptr, ws, P[0], P[1],
-                                                                  st, lane, uA, uB, energy);
+        pose_energrad<kMinimize, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, nullptr, ws, P[0],
+                                                                  P[1], st, lane, uA, uB, energy);
         const int i = acc_index(lane);
         if (!(lane & 3) && i < kAcc) {
           bs->acc[b][i] = uA;
@@ -1252,18 +1297,20 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
   const int lane = threadIdx.x % kWave;
   const int ntp = a.nrt * a.nlt;
   const int nslots = pair_slots(a.nrt, a.nlt);
-  // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | receptor [nrec + 1] |
+  // LDS: pair records [nslots + 1] | side records [side_slots(nslots)] | kh [kh_slots(nslots)] |
+  // receptor [nrec + 1] |
   // wave scratch [nwaves] | minimizer batches [nwaves], or pose matrices [nwaves][kBatch][12] and
   // (adjoint) the pair-table gradient [3][nslots], rows strided as the pair records
   float4* s_par = smem;
   float2* s_side = reinterpret_cast<float2*>(s_par + (nslots + 1));
-  float4* s_rec = reinterpret_cast<float4*>(s_side + side_slots(nslots));
+  float* s_kh = reinterpret_cast<float*>(s_side + side_slots(nslots));
+  float4* s_rec = reinterpret_cast<float4*>(s_kh + kh_slots(nslots));
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_mat = reinterpret_cast<float*>(s_ws + nwaves);
   // adjoint: one pair-table gradient [3][nslots] per wavefront
   float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * (kBatch * 12) : nullptr;
-  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, PLATEAU, EXACT, s_pgrad, nwaves * 3 * nslots);
+  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, s_kh, PLATEAU, EXACT, s_pgrad, nwaves * 3 * nslots);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
   float* w_pgrad = MODE == kAdjoint ? s_pgrad + wave * 3 * nslots : nullptr;
@@ -1274,7 +1321,7 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
 
   if (MODE == kMinimize) {
     BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
-    minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, ws, bs, st, lane);
+    minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, ws, bs, st, lane);
   } else {
     // a.batch poses of the processing order per work-counter claim; two consecutive ones of at
     // most kHalf beads share the bead phase.  Lane b fetches pose b's index, weight and matrix
@@ -1283,12 +1330,12 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
     float* Ms = s_mat + wave * (kBatch * 12);
     // the adjoint takes its claims in a fixed order (wavefront g of the grid: claims g, g + G,
     // g + 2G, ...), so each wavefront's table sums the same poses in the same order every run
-    const long long wave_id = (long long)blockIdx.x * nwaves + wave;
-    const long long stride_claims = (long long)gridDim.x * nwaves * a.batch;
-    for (long long claim = MODE == kAdjoint ? wave_id * a.batch : first_claim(a, lane); claim < a.N;
-         claim = MODE == kAdjoint ? claim + stride_claims : next_claim(a, lane)) {
-      const int k0 = (int)claim;
-      const int nbatch = min(a.batch, a.N - k0);
+    long long wave_id = (long long)blockIdx.x * nwaves + wave;
+    const long long stride_claims = (long long)gridDim.x * nwaves;
+    for (Claim claim = MODE == kAdjoint ? claim_of(a, wave_id) : first_claim(a, lane); claim.k0 < a.N;
+         claim = MODE == kAdjoint ? claim_of(a, wave_id += stride_claims) : next_claim(a, lane)) {
+      const int k0 = (int)claim.k0;
+      const int nbatch = claim.n;
       int p_l = 0;
       float w_l = 0.f;
       wave_sync();  // the previous claim's readers of Ms are done
@@ -1327,7 +1374,7 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
           }
         }
         float u[2];
-        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, w_pgrad, ws, P[0], P[1], st,
+        pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, w_pgrad, ws, P[0], P[1], st,
                                                              lane, u[0], u[1]);
         if (MODE != kAdjoint) {
           if (a.flags & ATJ_GRAD_QT) {
@@ -1928,12 +1975,22 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   const bool auto_blocks = blocks == 0;
   if (auto_blocks) blocks = resident;
   if (a.batch == 0) a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
+  if (a.tail_batch == 0 || a.tail_batch >= a.batch || MODE == kAdjoint) a.tail_batch = a.batch;
   if (auto_blocks) {  // no more blocks than the first round of claims needs
     const long long per_block = (long long)wpb * a.batch;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
   a.static_blocks = min(blocks, resident);
+  // the tail: the poses of the last tail_start rounds of claims (a.tail_start holds the
+  // rounds until here), claimed a.tail_batch at a time
+  if (a.tail_batch < a.batch) {
+    const long long round = (long long)a.static_blocks * wpb * a.batch;
+    const long long t = (long long)a.N - (long long)a.tail_start * round;
+    a.tail_start = t <= 0 ? 0 : (int)(t / a.batch * a.batch);
+  } else {
+    a.tail_start = a.N;
+  }
   if (MODE != kAdjoint) {  // the adjoint's claims are static: no work counter
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
     HIPCHK(hipGetLastError());
@@ -1973,13 +2030,20 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   if (MODE == kAdjoint) wpb = min(wpb, kMaxAdjointWaves);
   const uint32_t options = launch ? launch->options : 0u;
-  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_CLAIM_MASK)) return fail(ATJ_ERR_INVALID, "unknown launch option");
+  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_CLAIM_MASK | ATJ_LAUNCH_TAIL_MASK | ATJ_LAUNCH_TAIL_ROUNDS_MASK))
+    return fail(ATJ_ERR_INVALID, "unknown launch option");
   const int claim = (int)((options & ATJ_LAUNCH_CLAIM_MASK) >> 4);
   if (claim != 0 && claim != 1 && claim != 2 && claim != 4 && claim != kBatch)
     return fail(ATJ_ERR_INVALID, "poses per claim must be 1, 2, 4 or 8");
+  const int tail = (int)((options & ATJ_LAUNCH_TAIL_MASK) >> 8);
+  if (tail != 0 && tail != 1 && tail != 2 && tail != 4)
+    return fail(ATJ_ERR_INVALID, "poses per tail claim must be 1, 2 or 4");
+  const int tail_rounds = (int)((options & ATJ_LAUNCH_TAIL_ROUNDS_MASK) >> 12);
   a.batch = claim;  // 0: chosen at launch
+  a.tail_batch = tail;
+  a.tail_start = tail_rounds > 0 ? tail_rounds : 1;  // rounds until the launch shape is known
   const int nslots = pair_slots(a.nrt, a.nlt);
-  const size_t fixed = (size_t)(nslots + 1) * sizeof(float4) + side_slots(nslots) * sizeof(float2);
+  const size_t fixed = pair_lds_bytes(nslots);
   const size_t per_wave = sizeof(WaveScratch) +
                           (MODE == kMinimize ? sizeof(BatchScratch) : kBatch * 12 * sizeof(float)) +
                           (MODE == kAdjoint ? 3 * (size_t)nslots * sizeof(float) : 0);

@@ -321,6 +321,9 @@ def main():
                    "layout": "corner-packed" if system.grid.layout == _lib.ATJ_LAYOUT_CORNER else "plain",
                    "waves_per_block": system.launch.waves_per_block, "list_shells": system.grid.nshell,
                    "poses_per_claim": (system.launch.options >> 4) & 0xF or "auto",
+                   "tail_claims": ({"poses_per_claim": (system.launch.options >> 8) & 0xF,
+                                    "rounds": (system.launch.options >> 12) & 0xF or 1}
+                                   if system.launch.options & _lib.ATJ_LAUNCH_TAIL_MASK else None),
                    "receptor_path": "L1" if system.launch.options & _lib.ATJ_LAUNCH_REC_GLOBAL else "LDS"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -432,7 +435,8 @@ def main():
         specs, receptors = W.c5_stream(args.serve_jobs, rate=args.serve_rate)
         log(f"[bench] serving workload generated in {time.time() - t0:.1f}s ({len(receptors)} receptors)")
         be = GpuBackend(dev, receptors, waves_per_block=system.launch.waves_per_block,
-                        launch_options=system.launch.options & _lib.ATJ_LAUNCH_CLAIM_MASK)
+                        launch_options=system.launch.options & (_lib.ATJ_LAUNCH_CLAIM_MASK | _lib.ATJ_LAUNCH_TAIL_MASK |
+                                                                    _lib.ATJ_LAUNCH_TAIL_ROUNDS_MASK))
         budget = int(args.serve_budget_gib * 2 ** 30)
         sch = Scheduler(be, hbm_budget=budget, chunk_steps=25)
         # host time of each scheduler tick (admission builds grids on the host thread) and the

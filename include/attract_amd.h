@@ -79,11 +79,11 @@ typedef struct AtjGrid {
 
 /* ATTRACT pair table, compacted to receptor types x ligand types
  * (test-grid-energrad.py:67-79).  All arrays are float32 [nrt][nlt], row-major.  The pose
- * kernels stage it into each block's LDS as 24 bytes per slot, nrt x (nlt | 1) slots, next
+ * kernels stage it into each block's LDS as 28 bytes per slot, nrt x (nlt | 1) slots, next
  * to per-wavefront scratch (about 4.3 KB per wavefront for the minimizer, 3 KB for energrad;
  * the adjoint adds 12 bytes per slot).  A table that does not fit in 160 KB beside them
  * returns ATJ_ERR_UNSUPPORTED: at 16 wavefronts per block the minimizer takes up to about
- * 3,900 slots (61 receptor x 61 ligand types), fewer wavefronts per block leave room for more. */
+ * 3,400 slots (57 receptor x 57 ligand types), fewer wavefronts per block leave room for more. */
 typedef struct AtjForceField {
   const float* rc;
   const float* ac;
@@ -164,15 +164,24 @@ typedef struct AtjPoseState {
 #define ATJ_LAUNCH_CLAIM(n) ((uint32_t)(n) << 4)  /* poses per claim, n = 1, 2, 4 or 8 (bits 4-7;
                                                     0 = chosen from N and the wavefronts) */
 #define ATJ_LAUNCH_CLAIM_MASK 0xF0u
+#define ATJ_LAUNCH_TAIL(n) ((uint32_t)(n) << 8)  /* poses per claim in the tail, n = 1, 2 or 4 below
+                                                   the claim size (bits 8-11; 0 = no tail) */
+#define ATJ_LAUNCH_TAIL_MASK 0xF00u
+#define ATJ_LAUNCH_TAIL_ROUNDS(r) ((uint32_t)(r) << 12)  /* length of the tail: the last r rounds of
+                                                           claims, r = 1..15 (bits 12-15; 0 = 1) */
+#define ATJ_LAUNCH_TAIL_ROUNDS_MASK 0xF000u
 typedef struct AtjLaunch {
   int32_t waves_per_block;  /* 1..16 (0 = default 4); the adjoint uses at most 12 */
   int32_t blocks;           /* 0 = as many as the device holds, fewer for small launches */
   uint32_t options;         /* ATJ_LAUNCH_* */
 } AtjLaunch;
 /* Poses are claimed 1, 2, 4 or 8 at a time (chosen from N and the wavefronts launched, or set
- * by ATJ_LAUNCH_CLAIM); the
- * first claim of each wavefront in the blocks the device holds at once is fixed by its
- * index, further claims come from the work counter. */
+ * by ATJ_LAUNCH_CLAIM).  With ATJ_LAUNCH_TAIL the poses of the last rounds of claims (one round:
+ * a claim for every wavefront the device holds at once) are claimed in smaller claims, which
+ * even out the end of a persistent launch while the rest keeps the larger batches.  The first
+ * claim of each wavefront in the blocks the device holds at once is fixed by its index,
+ * further claims come from the work counter.  A pose's result does not depend on any of
+ * these options. */
 
 const char* atj_last_error(void);
 const char* atj_version(void);

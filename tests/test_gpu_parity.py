@@ -327,8 +327,9 @@ def test_c2_scale_subset(s2):
 
 def test_claim_sizes_give_identical_results(s1, s2):
     """On 32 wavefronts (8 blocks of 4), launches of 20, 40, 100 and 2048 poses claim 1, 2, 4 and
-    8 poses per atomic; every pose's energy, gradient and minimized pose are the same, bit for
-    bit, whatever the claim size."""
+    8 poses per atomic, and launches with a tail of smaller claims claim 8 and then 4 or 1;
+    every pose's energy, gradient and minimized pose are the same, bit for bit, whatever the
+    claims."""
     small = _lib.AtjLaunch(4, 8, 0)
     old2, old1 = s2.system.launch, s1.system.launch
     try:
@@ -346,6 +347,14 @@ def test_claim_sizes_give_identical_results(s1, s2):
             part = PoseState.create(w.quat[:n], w.trans[:n], DEV)
             minimize(s1.system, BatchPlan(s1.system, s1.ligs, [n], 12, w.lr_rot, w.lr_trans), part, 12)
             assert torch.equal(part.pose, full.pose[:n]) and torch.equal(part.energy, full.energy[:n]), n
+        for opt in (_lib.ATJ_LAUNCH_CLAIM(8) | _lib.ATJ_LAUNCH_TAIL(4, 2), _lib.ATJ_LAUNCH_CLAIM(8) |
+                    _lib.ATJ_LAUNCH_TAIL(1, 3), _lib.ATJ_LAUNCH_CLAIM(4) | _lib.ATJ_LAUNCH_TAIL(2, 15)):
+            s1.system.launch = s2.system.launch = _lib.AtjLaunch(4, 8, opt)
+            tail = PoseState.create(w.quat, w.trans, DEV)
+            minimize(s1.system, BatchPlan(s1.system, s1.ligs, [w.npose], 12, w.lr_rot, w.lr_trans), tail, 12)
+            assert torch.equal(tail.pose, full.pose) and torch.equal(tail.energy, full.energy), hex(opt)
+            En, Gn = s2.system.energrad(mat, s2.ligs[0])
+            assert torch.equal(En, E) and torch.equal(Gn, G), hex(opt)
     finally:
         s2.system.launch, s1.system.launch = old2, old1
 
