@@ -1976,17 +1976,23 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   if (auto_blocks) blocks = resident;
   if (a.batch == 0) a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
   if (a.tail_batch == 0 || a.tail_batch >= a.batch || MODE == kAdjoint) a.tail_batch = a.batch;
+  // a launch whose poses all fall in the tail (its rounds of claims cover them) is claimed
+  // tail_batch at a time from the start
+  const int tail_rounds = a.tail_start;
+  int first = a.batch;
+  if (a.tail_batch < a.batch &&
+      (long long)a.N <= (long long)tail_rounds * min(blocks, resident) * wpb * a.batch)
+    first = a.tail_batch;
   if (auto_blocks) {  // no more blocks than the first round of claims needs
-    const long long per_block = (long long)wpb * a.batch;
+    const long long per_block = (long long)wpb * first;
     const long long need = ((long long)a.N + per_block - 1) / per_block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
   }
   a.static_blocks = min(blocks, resident);
-  // the tail: the poses of the last tail_start rounds of claims (a.tail_start holds the
-  // rounds until here), claimed a.tail_batch at a time
+  // the tail: the poses of the last tail_rounds rounds of claims, claimed a.tail_batch at a time
   if (a.tail_batch < a.batch) {
     const long long round = (long long)a.static_blocks * wpb * a.batch;
-    const long long t = (long long)a.N - (long long)a.tail_start * round;
+    const long long t = (long long)a.N - (long long)tail_rounds * round;
     a.tail_start = t <= 0 ? 0 : (int)(t / a.batch * a.batch);
   } else {
     a.tail_start = a.N;
