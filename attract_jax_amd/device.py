@@ -40,6 +40,17 @@ def _dev(device):
     return torch.device(device if device is not None else "cuda")
 
 
+def _upload(a, device):
+    """A host array on the device, copied on the current stream from pinned memory: the
+    host does not wait for the copy, nor for work queued before it on the stream (a copy
+    from pageable memory would wait for both)."""
+    d = _dev(device)
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if d.type != "cuda":
+        return t.to(d)
+    return t.pin_memory().to(d, non_blocking=True)
+
+
 def pack_beads(xyz, tag):
     """float4 records (x, y, z, tag as int32 bits) as a float32 (n, 4) numpy array."""
     xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
@@ -62,7 +73,7 @@ class Receptor:
         if t.min(initial=0) < 0 or t.max(initial=0) > 0x7FFFFFFF:
             raise ValueError("receptor types out of range")
         b = pack_beads(coor_rec, t)
-        return cls(torch.from_numpy(b).to(_dev(device)), len(b), int(t.max(initial=0)))
+        return cls(_upload(b, device), len(b), int(t.max(initial=0)))
 
     def check_types(self, ff):
         """ValueError unless every bead's type is a row of the pair table ff."""
@@ -82,9 +93,11 @@ class Ligand:
     max_ff_type: int = None
 
     @classmethod
-    def create(cls, coor_lig, lig_atomtype_pos, lig_atomtypes, device=None):
+    def create(cls, coor_lig, lig_atomtype_pos, lig_atomtypes, device=None, stream=None):
         """coor_lig (nlig, 3) local coordinates; lig_atomtype_pos = potential-grid index of
-        every bead; lig_atomtypes = pair-table column of every bead."""
+        every bead; lig_atomtypes = pair-table column of every bead.  With a stream, the beads
+        are uploaded on it from pinned memory without waiting (a copy from pageable memory on
+        the default stream would wait for work on the other streams)."""
         gt = np.asarray(lig_atomtype_pos, np.int64)
         lt = np.asarray(lig_atomtypes, np.int64)
         if np.asarray(coor_lig).size == 0:
@@ -92,8 +105,15 @@ class Ligand:
         if gt.min(initial=0) < 0 or gt.max(initial=0) > 0xFFFF or lt.min(initial=0) < 0 or lt.max(initial=0) > 0x7FFF:
             raise ValueError("ligand types out of range")
         b = pack_beads(coor_lig, gt | (lt << 16))
-        return cls(torch.from_numpy(b).to(_dev(device)), len(b), np.asarray(coor_lig, np.float32).reshape(-1, 3),
-                   int(gt.max(initial=0)), int(lt.max(initial=0)))
+        if stream is None:
+            beads = torch.from_numpy(b).to(_dev(device))
+        else:
+            host = torch.from_numpy(b).pin_memory()
+            with torch.cuda.stream(stream):
+                beads = torch.empty(host.shape, dtype=host.dtype, device=_dev(device))
+                beads.copy_(host, non_blocking=True)
+        return cls(beads, len(b), np.asarray(coor_lig, np.float32).reshape(-1, 3), int(gt.max(initial=0)),
+                   int(lt.max(initial=0)))
 
     def check_types(self, grid, ff):
         """ValueError unless every bead's grid type is a potential grid of `grid` and its
@@ -135,8 +155,7 @@ class ForceField:
             rmin2 = 4.0 * rc64 / (3.0 * ac64)
         if not np.all(np.abs(np.asarray(ivor)) == 1):
             raise ValueError("ivor must be +1 or -1 (ATTRACT ipon)")
-        d = _dev(device)
-        f = lambda a: torch.as_tensor(np.ascontiguousarray(np.asarray(a, np.float64), ), dtype=torch.float32).to(d)
+        f = lambda a: _upload(np.asarray(np.asarray(a, np.float64), np.float32), device)
         return cls(f(rc), f(ac), f(ivor), f(emin), f(rmin2))
 
     @classmethod
@@ -348,6 +367,19 @@ class DeviceGrid:
         the potential of ligand grid type t sums nonbon over receptor beads within
         `cutoff`, capped at the plateau value inside plateaudis; lists hold the beads
         strictly within neighbourdis of each inner point, in receptor order."""
+        return cls.build_begin(receptor, ff, lig_fftype, origin=origin, spacing=spacing, dim=dim, dim2=dim2,
+                               gridextension=gridextension, plateaudis=plateaudis, neighbourdis=neighbourdis,
+                               cutoff=cutoff, layout=layout, device=device, keep_plain=keep_plain,
+                               potentials=potentials).finish()
+
+    @classmethod
+    def build_begin(cls, receptor, ff, lig_fftype, *, origin, spacing, dim, dim2=None, gridextension=0,
+                    plateaudis, neighbourdis, cutoff=50.0, layout=_lib.ATJ_LAYOUT_CORNER, device=None,
+                    keep_plain=False, potentials=True):
+        """The first half of build(), on the current stream without waiting for the device:
+        potentials and list counts are launched and the list total starts its copy to the
+        host.  Returns a GridBuild: ready() tells whether finish() can size the lists
+        without waiting; finish() launches the lists and returns the grid."""
         lib = _lib.load()
         d = _dev(device)
         ntypes = len(lig_fftype)
@@ -369,12 +401,12 @@ class DeviceGrid:
         self.inner_plain = self.outer_plain = None
         V = self.dim[0] * self.dim[1] * self.dim[2]
         self.nbgrid = torch.zeros((V, 2), dtype=torch.int32, device=d)
-        self.nblist = torch.full((LINE,), -1, dtype=torch.int16, device=d)  # replaced by the lists below
+        self.nblist = torch.full((LINE,), -1, dtype=torch.int16, device=d)  # replaced by the lists in finish()
         geom = self.struct()
         st = _stream(d)
         rstruct = receptor.struct()
         if potentials:
-            lt = torch.as_tensor(np.asarray(lig_fftype, np.int32)).to(d)
+            lt = _upload(np.asarray(lig_fftype, np.int32), d)
             inner = torch.empty((ntypes, *self.dim, 4), dtype=torch.float32, device=d)
             outer = torch.empty((ntypes, *self.dim2, 4), dtype=torch.float32, device=d) if dim2 is not None else None
             _lib.check(lib.atj_grid_build_potentials(ctypes.byref(geom), ctypes.byref(ff.struct()),
@@ -392,13 +424,24 @@ class DeviceGrid:
         slot = torch.where(c64 > 0, (c64 + LINE - 1) // LINE * LINE, torch.zeros_like(c64))
         ends = torch.cumsum(slot, 0) + LINE
         starts = (ends - slot).contiguous()
-        total = int(ends[-1].item()) if V else LINE
+        total = torch.full((1,), LINE, dtype=torch.int64).pin_memory()
+        if V:
+            total.copy_(ends[-1:], non_blocking=True)
+        counted = torch.cuda.Event()
+        counted.record(torch.cuda.current_stream(d))
+        return GridBuild(self, receptor, float(neighbourdis), counts, starts, total, counted,
+                         torch.cuda.current_stream(d))
+
+    def _finish_lists(self, receptor, neighbourdis, counts, starts, total):
+        lib = _lib.load()
+        d = self.device
+        V = self.dim[0] * self.dim[1] * self.dim[2]
         self.nblist = torch.empty(total, dtype=torch.int16, device=d)
         self.nblist[:LINE].fill_(-1)
         self.nbgrid = torch.empty((V, 2), dtype=torch.int32, device=d)
-        _lib.check(lib.atj_grid_fill_neighbours(ctypes.byref(self.struct()), ctypes.byref(rstruct),
+        _lib.check(lib.atj_grid_fill_neighbours(ctypes.byref(self.struct()), ctypes.byref(receptor.struct()),
                                                 float(neighbourdis), _ptr(counts), _ptr(starts),
-                                                _ptr(self.nbgrid), _ptr(self.nblist), st))
+                                                _ptr(self.nbgrid), _ptr(self.nblist), _stream(d)))
         return self
 
     def host_lists(self):
@@ -428,3 +471,22 @@ class DeviceGrid:
         if self.outer_plain is not None:
             d["outer_potential_grid"] = self.outer_plain.cpu().numpy()
         return namedtuple("Grid", d.keys())(*d.values())
+
+
+class GridBuild:
+    """A grid build between DeviceGrid.build_begin and finish (see there)."""
+
+    def __init__(self, grid, receptor, neighbourdis, counts, starts, total, counted, stream):
+        self.grid, self.receptor, self.neighbourdis = grid, receptor, neighbourdis
+        self.counts, self.starts, self.total, self.counted, self.stream = counts, starts, total, counted, stream
+
+    def ready(self):
+        """True once the list total has reached the host (finish() then waits for nothing)."""
+        return self.counted.query()
+
+    def finish(self):
+        """Size and fill the neighbour lists on the build's stream; returns the DeviceGrid."""
+        self.counted.synchronize()
+        with torch.cuda.stream(self.stream):
+            return self.grid._finish_lists(self.receptor, self.neighbourdis, self.counts, self.starts,
+                                           int(self.total[0]))

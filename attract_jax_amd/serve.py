@@ -255,10 +255,25 @@ class Scheduler:
             g.dirty = True
             self.groups.move_to_end(g.key)
 
-    def _run_set(self, now):
+    def _ready(self, g):
+        """Whether a group's grid is built (a backend without ready() builds at once).  A group
+        whose build failed has its jobs failed and is dropped."""
+        ready = getattr(self.backend, "ready", None)
+        if ready is None:
+            return True
+        try:
+            return ready(g)
+        except Exception as e:  # the build raised: its jobs cannot run
+            for j in list(g.jobs):
+                g.jobs.remove(j)
+                self._fail(j, f"grid build failed: {e}")
+            self._drop(g)
+            return False
+
+    def _run_set(self, now, groups):
         """Ids of the jobs to advance: jobs waiting longer than max_wait since admission,
         then the rest in order of remaining work, until their poses reach run_poses."""
-        runnable = [j for g in self.groups.values() for j in g.jobs if j.remaining > 0]
+        runnable = [j for g in groups for j in g.jobs if j.remaining > 0]
         runnable.sort(key=lambda j: (now - j.t_admit <= self.max_wait, j.remaining_work, j.id))
         chosen, poses = set(), 0
         for j in runnable:
@@ -269,12 +284,14 @@ class Scheduler:
         return chosen
 
     def _launch(self, now):
+        # groups whose grid is built; jobs of groups still building wait outside the run set
+        ready = [g for g in list(self.groups.values()) if g.jobs and self._ready(g)]
         chosen = None
-        for g in list(self.groups.values()):
-            if g.inflight is not None or not g.jobs:
+        for g in ready:
+            if g.inflight is not None:
                 continue
             if chosen is None:
-                chosen = self._run_set(now)
+                chosen = self._run_set(now, ready)
             jobs = [j for j in g.jobs if j.id in chosen]
             if not jobs:
                 continue
@@ -332,7 +349,11 @@ class GpuBackend:
     the receptor is made) or a dict {'grid_bytes', 'build'} whose build()
     returns a ScoringSystem.  The grid layout of a ReceptorSpec is chosen per group: corner-packed
     (8x the plain footprint, about twice the minimizer's speed) for jobs of at least
-    corner_min_work pose-steps when it fits, plain otherwise; layout= forces one."""
+    corner_min_work pose-steps when it fits, plain otherwise; layout= forces one.
+
+    A grid build never makes the scheduler wait: make_group launches its first half
+    (potentials, list counts) on the build stream, and ready() finishes it (the lists, sized by
+    the count) once the count has reached the host; a group launches once its grid is ready."""
 
     def __init__(self, device, receptors, layout=None, waves_per_block=16, corner_min_work=2e6, launch_options=0):
         import torch
@@ -345,12 +366,13 @@ class GpuBackend:
         self.options = launch_options
         # One stream per group, so that a host wait on a group's stream (result copy, plan
         # upload) waits for that group's work only; streams of dropped groups are reused.  Grids
-        # are built on the device's current stream, which no group launches on: the caching
-        # allocator keeps memory per stream, so grids built there share one pool of cached
-        # memory however many group streams there are.
+        # are built on one build stream, which no group launches on: the caching allocator keeps
+        # memory per stream, so grids built there share one pool of cached memory however many
+        # group streams there are.
         self._free_streams = []
+        self._build_stream = torch.cuda.Stream(self.device)
         self._built = {}
-        self.build_seconds = []       # host time of each grid build (its launches and syncs)
+        self.build_seconds = []       # time from each grid build's start to its grid being ready
         self.layouts = []             # layout of each group made
 
     def grid_bytes(self, key):
@@ -369,33 +391,37 @@ class GpuBackend:
         work = 0 if job is None else job.npose * job.spec.nsteps
         return [corner, plain] if work >= self.corner_min_work else [plain]
 
-    def _system(self, key, layout=None):
-        if key not in self._built:
-            r = self.receptors[key]
+    def _build_begin(self, key, layout):
+        """Start building the receptor's scoring system on the build stream: a finished
+        system, or (receptor, pair table, GridBuild) to finish in _build_finish."""
+        torch = self.torch
+        system = self._built.get(key)
+        if system is not None:
+            return system
+        r = self.receptors[key]
+        with torch.cuda.stream(self._build_stream):
             if isinstance(r, dict):
-                self._built[key] = r["build"]()
-            else:
-                from .device import DeviceGrid, ForceField, Receptor
-                from .score import ScoringSystem
-                from .workloads import NTYPES
-                ff = ForceField.create(r.rc, r.ac, r.ivor, device=self.device)
-                rec = Receptor.create(r.coor, r.types - 1, self.device)
-                grid = DeviceGrid.build(rec, ff, np.arange(NTYPES), layout=layout, device=self.device, **r.grid_kw)
-                self._built[key] = ScoringSystem(grid, ff, rec)
-        return self._built[key]
+                return r["build"]()
+            from .device import DeviceGrid, ForceField, Receptor
+            from .workloads import NTYPES
+            ff = ForceField.create(r.rc, r.ac, r.ivor, device=self.device)
+            rec = Receptor.create(r.coor, r.types - 1, self.device)
+            return rec, ff, DeviceGrid.build_begin(rec, ff, np.arange(NTYPES), layout=layout, device=self.device,
+                                                   **r.grid_kw)
+
+    def _build_finish(self, key, pending):
+        from .score import ScoringSystem
+        with self.torch.cuda.stream(self._build_stream):
+            rec, ff, gb = pending
+            system = ScoringSystem(gb.finish(), ff, rec)
+        self._built[key] = system
+        return system
 
     def make_group(self, g):
-        from . import _lib
         from .minimize import PoseState
         torch = self.torch
         cap = g.pool.capacity
         stream = self._free_streams.pop() if self._free_streams else torch.cuda.Stream(self.device)
-        t0 = time.perf_counter()
-        system = self._system(g.key, g.layout)
-        self.build_seconds.append(time.perf_counter() - t0)
-        self.layouts.append(system.grid.layout)
-        system.launch = _lib.AtjLaunch(self.waves, 0, self.options)
-        stream.wait_stream(torch.cuda.current_stream(self.device))  # the grid's build
         # the group's arrays belong to its stream: the caching allocator then reuses their
         # memory only in that stream's order once they are dropped (grow_group, drop_group)
         z = torch.zeros
@@ -404,7 +430,32 @@ class GpuBackend:
                               z((cap, 7), device=self.device), z(cap, dtype=torch.int32, device=self.device),
                               z(cap, device=self.device))
             counter = torch.zeros(1, dtype=torch.int32, device=self.device)
-        g.handle = dict(system=system, state=state, stream=stream, counter=counter, plan=None)
+        g.handle = dict(system=None, state=state, stream=stream, counter=counter, plan=None,
+                        build=self._build_begin(g.key, g.layout), t_build=time.perf_counter())
+        self.ready(g)
+
+    def ready(self, g, wait=False):
+        """True once the group's grid is built (wait: finish it now, waiting for the device).
+        A failed build raises here."""
+        from . import _lib
+        from .score import ScoringSystem
+        h = g.handle
+        if h["system"] is None:
+            b = h["build"]
+            if isinstance(b, ScoringSystem):
+                system = b
+            else:
+                if not wait and not b[2].ready():
+                    return False
+                system = self._build_finish(g.key, b)
+            system.launch = _lib.AtjLaunch(self.waves, 0, self.options)
+            done = self.torch.cuda.Event()
+            done.record(self._build_stream)
+            h["stream"].wait_event(done)  # the group's launches follow the grid's build
+            h["system"], h["build"] = system, None
+            self.build_seconds.append(time.perf_counter() - h["t_build"])
+            self.layouts.append(system.grid.layout)
+        return True
 
     def grow_group(self, g, new_capacity):
         torch = self.torch
@@ -437,6 +488,10 @@ class GpuBackend:
         the next build at once).  The memory stays with the caching allocator for the next
         grids: emptying the cache would wait for every running launch and make the next grid
         a fresh device allocation."""
+        try:  # a group whose job failed to load may still be building
+            self.ready(g, wait=True)
+        except Exception:  # a failed build leaves nothing to keep
+            pass
         self._built.pop(g.key, None)
         self._free_streams.append(g.handle["stream"])
         g.handle = None
@@ -448,8 +503,18 @@ class GpuBackend:
         torch = self.torch
         h = g.handle
         sp = job.spec
-        lig = Ligand.create(sp.ligand_coor, sp.ligand_types - 1, sp.ligand_types - 1, self.device)
-        lig.check_types(h["system"].grid, h["system"].ff)
+        lig = Ligand.create(sp.ligand_coor, sp.ligand_types - 1, sp.ligand_types - 1, self.device, stream=h["stream"])
+        r = self.receptors[g.key]
+        if h["system"] is None and not isinstance(r, dict):  # the grid is still building
+            from .workloads import NTYPES
+            if lig.max_grid_type is not None and lig.max_grid_type >= NTYPES:
+                raise ValueError(f"ligand grid type {lig.max_grid_type} beyond the {NTYPES} potential grids")
+            if lig.max_ff_type is not None and lig.max_ff_type >= np.shape(r.rc)[1]:
+                raise ValueError(f"ligand type {lig.max_ff_type} beyond the {np.shape(r.rc)[1]} columns of the "
+                                 f"pair table")
+        else:
+            self.ready(g, wait=True)
+            lig.check_types(h["system"].grid, h["system"].ff)
         sl = slice(slot, slot + job.npose)
         with torch.cuda.stream(h["stream"]):
             pose = torch.cat([torch.as_tensor(sp.quat, dtype=torch.float32),
@@ -470,7 +535,8 @@ class GpuBackend:
         for i, j in enumerate(jobs):
             opt = _lib.AtjOptimizer(_lib.ATJ_OPT_ADAM, j.spec.lr_rot, j.spec.lr_trans, 0.9, 0.999, 1e-8, 0.0, 0.0)
             arr[i] = _lib.AtjJob(ligs[i].struct(), j.slot, j.npose, j.spec.nsteps, 0, opt)
-        jobs_t = torch.from_numpy(np.frombuffer(bytes(arr), np.uint8).copy()).to(self.device)
+        from .device import _upload  # pinned and non-blocking: the launch never waits on the host
+        jobs_t = _upload(np.frombuffer(bytes(arr), np.uint8).copy(), self.device)
         cap = g.handle["state"].N
         pose_job = np.zeros(cap, np.int32)
         order = []
@@ -478,8 +544,8 @@ class GpuBackend:
             pose_job[j.slot:j.slot + j.npose] = i
             order.append(np.arange(j.slot, j.slot + j.npose, dtype=np.int32))
         order = np.concatenate(order)
-        return dict(jobs=jobs_t, njobs=len(jobs), pose_job=torch.from_numpy(pose_job).to(self.device),
-                    order=torch.from_numpy(order).to(self.device), N=len(order))
+        return dict(jobs=jobs_t, njobs=len(jobs), pose_job=_upload(pose_job, self.device),
+                    order=_upload(order, self.device), N=len(order))
 
     def launch(self, g, jobs, steps, dirty):
         import ctypes

@@ -71,8 +71,7 @@ def test_served_job_matches_twin():
     s, _ = run(rs, [sp], chunk=15)
     res = s.jobs[0].result
     r = rs[sp.receptor_key]
-    be = GpuBackend(DEV, W.c5_backend_receptors(rs, DEV, keep_plain=True))
-    system = be._system(sp.receptor_key)
+    system = W.c5_backend_receptors(rs, DEV, keep_plain=True)[sp.receptor_key]["build"]()
     grid = system.grid.to_reference_grid()
     ffa = [t.cpu().numpy() for t in (system.ff.rc, system.ff.ac, system.ff.ivor, system.ff.emin, system.ff.rmin2)]
     tw = Twin(grid, ffa, r["coor"], r["types"] - 1, sp.ligand_coor, sp.ligand_types - 1, sp.ligand_types - 1)

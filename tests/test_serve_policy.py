@@ -323,3 +323,44 @@ def test_evictions_and_admission_waits_are_counted():
     assert s.evictions >= 2
     waits = s.admission_waits()
     assert len(waits) == 4 and waits.max() > 0
+
+
+class BuildingBackend(FakeBackend):
+    """Grids take `build_ticks` calls of ready() to build; keys in `broken` fail their build."""
+
+    def __init__(self, clock, grid_bytes, build_ticks=5, broken=()):
+        super().__init__(clock, grid_bytes)
+        self.build_ticks = build_ticks
+        self.broken = set(broken)
+
+    def make_group(self, g):
+        g.handle = {"left": self.build_ticks}
+
+    def ready(self, g):
+        if g.key in self.broken:
+            raise RuntimeError("no memory for the grid")
+        g.handle["left"] -= 1
+        return g.handle["left"] < 0
+
+
+def test_groups_launch_once_their_grid_is_built():
+    """A group launches only after its grid is ready, and meanwhile another receptor's jobs
+    run; jobs of a receptor whose build fails are failed with the reason, the group is
+    dropped and later jobs run."""
+    clock = FakeClock()
+    be = BuildingBackend(clock, {"a": 100, "b": 100, "bad": 100}, build_ticks=5, broken={"bad"})
+    s = Scheduler(be, hbm_budget=10 ** 9, chunk_steps=10, initial_slots=16, clock=clock)
+    s.submit(spec("a", 8, 30))
+    s.tick()
+    for _ in range(3):
+        clock.t += 1e-3
+        s.tick()
+    assert be.launches == []  # a is still building
+    s.submit(spec("bad", 8, 30))
+    s.submit(spec("b", 8, 30))
+    drive(s, clock)
+    failed = [j for j in s.done if j.state == "failed"]
+    assert [j.spec.receptor_key for j in failed] == ["bad"] and "grid build failed" in failed[0].error
+    assert "bad" in be.dropped and "bad" not in s.groups
+    assert sorted(j.spec.receptor_key for j in s.done if j.state == "done") == ["a", "b"]
+    assert be.launches[0][0] == "a"
