@@ -104,9 +104,13 @@ def minimize(system, plan, state, nsteps_launch, flags=None, stats=False, stream
 
 
 def minimize_plain(system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8,
-                   flags=None, stream=None, jobs=None, *, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
+                   flags=None, stream=None, jobs=None, *, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0,
+                   launch=None):
     """Per-job form: for each step, one energrad launch (quaternion poses) then one
-    optimizer launch; pose state lives in HBM between launches."""
+    optimizer launch; pose state lives in HBM between launches.  The energrad launches use
+    `launch`, by default the system's waves per block and receptor path with the claim size
+    left to each launch: claim options tuned for a persistent minimization (many poses per
+    wavefront) would leave most of the device idle in these one-step launches."""
     lib = _lib.load()
     d = system.device
     st = stream if stream is not None else _stream(d)
@@ -115,13 +119,16 @@ def minimize_plain(system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, b
         jobs = system.jobs_tensor([ligand])
     dEdM = torch.empty((N, 16), dtype=torch.float32, device=d)
     fl = ((system.flags if flags is None else flags) | _lib.ATJ_POSE_QT)
+    if launch is None:
+        sl = system.launch
+        launch = _lib.AtjLaunch(sl.waves_per_block, sl.blocks, sl.options & _lib.ATJ_LAUNCH_REC_GLOBAL)
     ss = state.struct()
     opt = _lib.AtjOptimizer(int(optimizer), float(lr_rot), float(lr_trans), float(beta1), float(beta2), float(eps),
                             float(eps_root), float(gravity))
     for _ in range(int(nsteps)):
         _lib.check(lib.atj_energrad(ctypes.byref(system._gs), ctypes.byref(system._fs), ctypes.byref(system._rs),
                                     _ptr(jobs), 1, ctypes.c_void_p(0), ctypes.c_void_p(0), _ptr(state.pose), N,
-                                    _ptr(state.energy), _ptr(dEdM), fl, ctypes.byref(system.launch),
+                                    _ptr(state.energy), _ptr(dEdM), fl, ctypes.byref(launch),
                                     _ptr(system.counter), ctypes.c_void_p(0), st))
         _lib.check(lib.atj_optimizer_step(_ptr(dEdM), _ptr(state.energy), N, ctypes.byref(ss), ctypes.byref(opt), st))
     return state

@@ -372,6 +372,8 @@ def main():
         plain = pst / tp
         result["plain_pose_steps_per_s"] = plain
         result["plain_sample"] = f"{nl} of {len(ligs)} ligands x {w.pose_counts[0]} poses x {w.nsteps} steps"
+        result["plain_launch"] = (f"{system.launch.waves_per_block} waves per block, poses per claim chosen by "
+                                  f"each one-step launch (the minimizer's claim options left out)")
         result["batched_vs_plain"] = value / plain
         # same poses: batched result vs plain result, every ligand timed
         worst = 0.0
