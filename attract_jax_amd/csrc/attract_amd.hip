@@ -2029,9 +2029,6 @@ template <int MODE>
 int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   if (!a.counter && MODE != kAdjoint) return fail(ATJ_ERR_INVALID, "work counter is NULL");
   if (!a.jobs) return fail(ATJ_ERR_INVALID, "jobs is NULL");
-  DevInfo d;
-  int rc = device_info(d);
-  if (rc) return rc;
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
   if (MODE == kAdjoint) wpb = min(wpb, kMaxAdjointWaves);
@@ -2048,6 +2045,9 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   a.batch = claim;  // 0: chosen at launch
   a.tail_batch = tail;
   a.tail_start = tail_rounds > 0 ? tail_rounds : 1;  // rounds until the launch shape is known
+  DevInfo d;
+  int rc = device_info(d);
+  if (rc) return rc;
   const int nslots = pair_slots(a.nrt, a.nlt);
   const size_t fixed = pair_lds_bytes(nslots);
   const size_t per_wave = sizeof(WaveScratch) +

@@ -50,11 +50,11 @@ def _rec(n=10):
     return _lib.AtjReceptor(_addr(4 * n), n)
 
 
-def _energrad(lib, grid, ff=None, rec=None, N=4, poses=True, energy=True, flags=None):
+def _energrad(lib, grid, ff=None, rec=None, N=4, poses=True, energy=True, flags=None, launch=None):
     ff = ff or _ff()
     rec = rec or _rec()
     flags = _lib.ATJ_TERM_GRID | _lib.ATJ_TERM_NB if flags is None else flags
-    launch = _lib.AtjLaunch(16, 0, 0)
+    launch = launch or _lib.AtjLaunch(16, 0, 0)
     return lib.atj_energrad(ctypes.byref(grid) if grid is not None else None, ctypes.byref(ff), ctypes.byref(rec),
                             _addr(), 1, None, None, _addr() if poses else None, N, _addr() if energy else None,
                             None, flags, ctypes.byref(launch), _addr(dtype=np.int32), None, None)
@@ -144,3 +144,16 @@ def test_minimize_counts(lib):
                           ctypes.byref(state), 0, _lib.ATJ_TERM_GRID | _lib.ATJ_TERM_NB, ctypes.byref(launch),
                           _addr(dtype=np.int32), None, None)
     assert st == 0  # zero steps: nothing to do
+
+
+def test_launch_options(lib):
+    """Waves per block, unknown option bits and claim sizes are refused before any device call."""
+    L = lambda w, o: _lib.AtjLaunch(w, 0, o)
+    assert _energrad(lib, _grid(), launch=L(17, 0)) == INVALID and "waves_per_block" in _msg(lib)
+    assert _energrad(lib, _grid(), launch=L(16, 1 << 20)) == INVALID and "unknown launch option" in _msg(lib)
+    for n in (3, 5, 6, 15):
+        assert _energrad(lib, _grid(), launch=L(16, _lib.ATJ_LAUNCH_CLAIM(n))) == INVALID, n
+        assert "per claim" in _msg(lib)
+    for n in (3, 8):
+        assert _energrad(lib, _grid(), launch=L(16, _lib.ATJ_LAUNCH_CLAIM(8) | _lib.ATJ_LAUNCH_TAIL(n))) == INVALID, n
+        assert "tail claim" in _msg(lib)
