@@ -1630,7 +1630,9 @@ __device__ void stage_by_type(const float4* rec, int c0, int n, int nrt, float4*
   }
 }
 
-__global__ void __launch_bounds__(kBuildThreads)
+// Three blocks per CU (three wavefronts per SIMD, at most 168 VGPRs): the bead loop waits on
+// LDS, and the 128 output accumulators leave no room for a fourth.
+__global__ void __launch_bounds__(kBuildThreads, 3)
     build_potentials_kernel(GridView G, const float4* rec, int nrec, const float* rc, const float* ac,
                             const float* ivor, const float* emin, const float* rmin2, int nrt, int nlt,
                             const int* lig_fftype, int ntypes, float cutoff2, float* out, int dx, int dy, int dz,
@@ -1669,6 +1671,8 @@ __global__ void __launch_bounds__(kBuildThreads)
       const int b0 = s_off[rt], b1 = s_off[rt + 1];
       if (b0 == b1) continue;
       float U4 = 0.f, U3 = 0.f, V5[3] = {0.f, 0.f, 0.f}, V4[3] = {0.f, 0.f, 0.f};
+      // unrolled so that several LDS reads of beads are in flight at once
+#pragma unroll 4
       for (int b = b0; b < b1; ++b) {
         const float4 rb = s_bead[b];
         const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
@@ -1676,10 +1680,13 @@ __global__ void __launch_bounds__(kBuildThreads)
         const bool in = dsq < cutoff2;
         const bool capped = dsq < P2;
         const float dd = capped ? P2 : dsq;
-        const float u = 1.f / dd;
+        // hardware reciprocal and reciprocal square root (1 ulp): the grid is data compared
+        // with the FP64 restatement at 1e-4, and IEEE division and square root were most of
+        // this loop's instructions
+        const float u = __builtin_amdgcn_rcpf(dd);
         const float u3 = (u * u) * u;
         const float u4 = u3 * u;
-        const float c = capped ? (dsq > 0.f ? P / sqrtf(dsq) : 0.f) : 1.f;
+        const float c = capped ? (dsq > 0.f ? P * __builtin_amdgcn_rsqf(dsq) : 0.f) : 1.f;
         const float w4 = in ? 2.f * c * u4 : 0.f;
         const float w5 = w4 * u;
         U4 += in ? u4 : 0.f;
@@ -1714,11 +1721,11 @@ __global__ void __launch_bounds__(kBuildThreads)
             const bool capped = dsq < P2;
             const float dd = capped ? P2 : dsq;
             if (!(dsq < cutoff2) || !(dd < r2)) continue;
-            const float u = 1.f / dd;
+            const float u = __builtin_amdgcn_rcpf(dd);
             const float u3 = (u * u) * u;
             const float vlj = (crc * u - cac) * u3;
             const float dv = -(fmaf(4.f, crc * u, -3.f * cac) * u3) * u;
-            const float c = capped ? (dsq > 0.f ? P / sqrtf(dsq) : 0.f) : 1.f;
+            const float c = capped ? (dsq > 0.f ? P * __builtin_amdgcn_rsqf(dsq) : 0.f) : 1.f;
             const float gs = 2.f * (1.f - iv) * dv * c;
             E[k] += (1.f - iv) * (vlj - em);
             Fx[k] = fmaf(-gs, ddx, Fx[k]);
