@@ -113,7 +113,8 @@ def grid_build_times(dev, best_ms, stream):
 def _child_command(args, wpb, opt):
     """A one-launch run of this workload that times nothing else (the profiled child)."""
     return [sys.executable, "-u", os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-            "--no-plain", "--serve-jobs", "0", "--no-traffic", "--no-scoring", "--copy-calibration", "--waves",
+            "--no-plain", "--no-share", "--serve-jobs", "0", "--no-traffic", "--no-scoring", "--copy-calibration",
+            "--waves",
             str(wpb), "--launch-options", str(opt), "--ligands", str(args.ligands), "--poses-per-ligand",
             str(args.poses_per_ligand), "--adam-steps", str(args.adam_steps), "--nshell", str(args.nshell)]
 
@@ -197,6 +198,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-share", action="store_true", help="skip one GPU's share of the workload on eight GPUs")
     ap.add_argument("--waves", type=int, default=0, help="waves per block (0 = autotuned)")
     ap.add_argument("--launch-options", type=int, default=0, help="ATJ_LAUNCH_* bits when --waves is given")
     ap.add_argument("--nshell", type=int, default=6, help="distance shells of the neighbour lists (0 = whole lists)")
@@ -335,6 +337,33 @@ def main():
                               "est_tflops": flops / (k_ms * 1e-3) / 1e12},
         "energy_finite_frac": float(np.isfinite(E_final).mean()),
     }
+
+    if not args.no_share:
+        # one GPU's share of the workload on eight GPUs (poses sharded, grid replicated): the first
+        # eighth of every ligand's poses, one launch of all steps, best of three
+        counts = [max(1, c // 8) for c in w.pose_counts]
+        idx = np.concatenate([np.arange(plan.offsets[k], plan.offsets[k] + counts[k]) for k in range(len(counts))])
+        sub = BatchPlan(system, ligs, counts, w.nsteps, w.lr_rot, w.lr_trans)
+        sst = PoseState.create(w.quat[idx], w.trans[idx], dev)
+        sinit = sst.pose.clone()
+        ms = []
+        for _ in range(4):
+            sst.pose.copy_(sinit)
+            sst.m.zero_()
+            sst.v.zero_()
+            sst.count.zero_()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            minimize(system, sub, sst, w.nsteps)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        share_ms = min(ms[1:])
+        result["one_gpu_share_of_8"] = {
+            "poses": int(sub.N), "kernel_ms": share_ms, "pose_steps_per_s": sub.N * w.nsteps / (share_ms * 1e-3),
+            "vs_full_rate": (sub.N * w.nsteps / (share_ms * 1e-3)) / (pose_steps / (k_ms * 1e-3))}
+        log(f"[bench] one GPU's share of eight ({sub.N} poses): {share_ms:.2f} ms, "
+            f"{result['one_gpu_share_of_8']['vs_full_rate']:.2f} of the full-size rate")
 
     if not args.no_traffic:
         tr = measure_traffic(args, system.launch.waves_per_block, system.launch.options)
