@@ -119,6 +119,7 @@ def ATJ_LAUNCH_CLAIM(n):
 
 ATJ_LAUNCH_TAIL_MASK = 0xF00
 ATJ_LAUNCH_TAIL_ROUNDS_MASK = 0xF000
+ATJ_LAUNCH_ADJOINT_WAVE_TABLES = 0x10000  # adjoint: per-wavefront float tables, not the fixed-point one
 
 
 def ATJ_LAUNCH_TAIL(n, rounds=1):

@@ -79,14 +79,14 @@ def param_adjoint(system, ligand, M, ff, weights, flags, blocks=None, jobs=None,
     lib = _lib.load()
     d = system.device
     N = M.shape[0]
-    if blocks is None:  # one block per CU: a block's wavefront tables fill most of its LDS
+    if blocks is None:  # one block per CU: a block's gradient tables fill most of its LDS
         blocks = torch.cuda.get_device_properties(d).multi_processor_count
     ntp = ff.nrt * ff.nlt
     partial = torch.empty((blocks, 3 * ntp), dtype=torch.float32, device=d)
     out = torch.empty(3 * ntp, dtype=torch.float32, device=d)
     if jobs is None:
         jobs = system.jobs_tensor([ligand])
-    launch = _lib.AtjLaunch(waves_per_block or 12, blocks,
+    launch = _lib.AtjLaunch(waves_per_block or 16, blocks,
                             system.launch.options if options is None else options)
     _lib.check(lib.atj_energrad_adjoint(ctypes.byref(system._gs), ctypes.byref(ff.struct()),
                                         ctypes.byref(system._rs), _ptr(jobs), 1, ctypes.c_void_p(0), _ptr(M), N,

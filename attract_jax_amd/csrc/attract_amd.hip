@@ -55,14 +55,14 @@ constexpr int kHalf = 32;          // ligands of at most this many beads take ha
 constexpr int kMaxShell = 6;       // most list shells per grid point (16-bit counts in one int4)
 constexpr double kShellMargin = 1e-3;  // A added to every shell radius: covers FP32 rounding of distances
 constexpr int kMaxWavesPerBlock = 16;
-// The adjoint's pair loop holds more values (the LDS gradient indices and terms): its blocks
-// have at most 12 wavefronts, 3 per SIMD, so that it may use up to 168 VGPRs without spills.
-constexpr int kMaxAdjointWaves = 12;
 constexpr int kAcc = 13;           // E, dE/dM[0..2][0..2], dE/dM[3][0..2]
 constexpr int kPairCost = 30;      // approximate instructions per neighbour slot
 constexpr int kItemCost = 90;      // approximate instructions per list segment
 
-enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2 };
+// kAdjoint sums the pair-table gradient in one fixed-point table per block; kAdjointWave in
+// float tables of each wavefront's own (the form for pair tables too large for the first).
+enum Mode { kEnergrad = 0, kMinimize = 1, kAdjoint = 2, kAdjointWave = 3 };
+__host__ __device__ constexpr bool is_adjoint(int mode) { return mode >= kAdjoint; }
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -482,6 +482,51 @@ __device__ __forceinline__ void pair_param_grad(float dsq, float u, const float4
   d[2] = lo && neg ? -2.f : 0.f;  // ivor - 1
 }
 
+// Fixed-point sums of the pair-table gradient.  A sum is kFixWords signed 64-bit words w[c]
+// (c = 0, 1, 2), worth sum_c w[c] 2^(32c - kFixFrac).  A term is rounded to a multiple of
+// 2^-kFixFrac; its 24-bit significand, shifted to its place, straddles at most two words, and
+// each word takes a signed addend below 2^32, so 2^31 adds per word cannot overflow.  Integer
+// adds give the same sum in any order, and LDS integer atomics are much faster than float ones
+// (ds_add_u64 ~17x the rate of ds_add_f32 on gfx950).
+constexpr int kFixWords = 3;
+constexpr int kFixFrac = 48;  // resolution 2^-48; terms below 2^48 in magnitude are summed exactly
+
+// Adds v to the fixed-point sum whose words are t[0], t[stride], t[2 stride].  Terms of 2^48
+// and above, infinities and NaNs are added to the float *big instead.
+__device__ __forceinline__ void fixed_add(unsigned long long* t, unsigned stride, float* big, float v) {
+  const unsigned bits = __float_as_uint(v);
+  // v = +-m 2^(ex - 150) (zero and subnormals too: they round to 0 below); p is the place of
+  // m's lowest bit in units of 2^-kFixFrac
+  const int p = (int)__builtin_amdgcn_ubfe(bits, 23, 8) + (kFixFrac - 150);
+  if (p > 32 * kFixWords - 24) {
+    atomicAdd(big, v);
+    return;
+  }
+  // below the resolution: round the magnitude to the nearest unit (halves away from zero); a
+  // shift of 25 or more leaves 0, as m < 2^24
+  const unsigned s = (unsigned)min(max(-p, 0), 31);
+  const unsigned m = (((bits & 0x7fffffu) | 0x800000u) + ((1u << s) >> 1)) >> s;
+  const unsigned q = (unsigned)max(p, 0);
+  const unsigned long long x = (unsigned long long)m << (q & 31u);
+  const unsigned neg = (unsigned)((int)bits >> 31);  // all ones for v < 0: two's complement negation
+  unsigned long long* w = t + __umul24(q >> 5, stride);
+  const unsigned long long neg64 = (unsigned long long)(long long)(int)neg;
+  atomicAdd(w, ((x & 0xffffffffull) ^ neg64) - neg64);
+  const unsigned hi = (unsigned)(x >> 32);  // below 2^25
+  if (hi) atomicAdd(w + stride, (unsigned long long)(long long)(int)((hi ^ neg) - neg));
+}
+
+// The value of a fixed-point sum, rounded to float: carries are propagated so that the two
+// lower words lie in [0, 2^32), then the parts are added in double from the bottom up.
+__device__ __forceinline__ float fixed_value(const unsigned long long* t, unsigned stride) {
+  long long w0 = (long long)t[0], w1 = (long long)t[stride], w2 = (long long)t[2 * stride];
+  w1 += w0 >> 32;
+  w2 += w1 >> 32;
+  const double r0 = (double)(w0 & 0xffffffffll), r1 = (double)(w1 & 0xffffffffll);
+  const double low = fma(r1, 0x1p-16, r0 * 0x1p-48);
+  return (float)fma((double)w2, 0x1p16, low);
+}
+
 // LDS pair-table rows (one per receptor type) have an odd stride of nlt | 1 records, so
 // the records of one ligand type under different receptor types fall in different LDS
 // banks (lanes working on one ligand bead read one column).  Slot nslots holds the
@@ -492,6 +537,9 @@ __host__ __device__ __forceinline__ int pair_slots(int nrt, int nlt) { return nr
 // multiple of 4, keeping what follows 16-byte aligned.
 __host__ __device__ __forceinline__ int side_slots(int nslots) { return (nslots + 2) & ~1; }
 __host__ __device__ __forceinline__ int kh_slots(int nslots) { return (nslots + 4) & ~3; }
+// Floats of kAdjoint's block table: kFixWords 64-bit words per sum, then a float per sum for
+// large terms, over [3][nslots] sums.
+__host__ __device__ __forceinline__ int adjoint_table_floats(int nslots) { return (2 * kFixWords + 1) * 3 * nslots; }
 // LDS bytes of the staged pair-type records.
 __host__ __device__ __forceinline__ size_t pair_lds_bytes(int nslots) {
   return (size_t)(nslots + 1) * 16 + (size_t)side_slots(nslots) * 8 + (size_t)kh_slots(nslots) * 4;
@@ -676,15 +724,17 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         const unsigned pb = (ok[cc] ? (REC_LDS ? row : __umul24(row, row_bytes)) : zrow) + col;
         pidx[cc] = pb;
         pa[cc] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_par) + pb);
-        ps[cc] = ENERGY && MODE != kAdjoint ? s_side[pb >> 4] : make_float2(0.f, 0.f);
-        kh[cc] = PLATEAU && ENERGY && MODE != kAdjoint ? s_kh[pb >> 4] : 0.f;
+        ps[cc] = ENERGY && !is_adjoint(MODE) ? s_side[pb >> 4] : make_float2(0.f, 0.f);
+        kh[cc] = PLATEAU && ENERGY && !is_adjoint(MODE) ? s_kh[pb >> 4] : 0.f;
       }
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
         npair += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(ok[cc]));
-        if (MODE == kAdjoint) {
-          // only the parameter derivatives, added to this wavefront's own table: no other
-          // wavefront writes it, and its adds come in a fixed order, so sums are reproducible
+        if (is_adjoint(MODE)) {
+          // only the parameter derivatives.  kAdjoint: into the block's fixed-point table,
+          // whose sums do not depend on the order of the adds.  kAdjointWave: into this
+          // wavefront's own float table, which no other wavefront writes and whose adds come
+          // in a fixed order.  Either way reruns give the same sums.
           if (ok[cc]) {
             float dp[3];
             pair_param_grad(dsq[cc], __builtin_amdgcn_rcpf(dsq[cc]), pa[cc], dp);
@@ -696,10 +746,20 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
               dp[2] -= dq[2];
             }
             const unsigned pg = pidx[cc] >> 4;
-            atomicAdd(&s_pgrad[pg], wgt * dp[0]);
-            atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
-            // the emin derivative is zero except for repulsive pairs inside rmin
-            if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
+            if (MODE == kAdjoint) {
+              // words [kFixWords][3][nslots], then the float table of large terms [3][nslots]
+              unsigned long long* t = reinterpret_cast<unsigned long long*>(s_pgrad);
+              float* big = s_pgrad + 2 * kFixWords * 3 * nslots;
+              const unsigned stride = 3 * nslots;
+              fixed_add(t + pg, stride, big + pg, wgt * dp[0]);
+              fixed_add(t + nslots + pg, stride, big + nslots + pg, wgt * dp[1]);
+              // the emin derivative is zero except for repulsive pairs inside rmin
+              if (dp[2] != 0.f) fixed_add(t + 2 * nslots + pg, stride, big + 2 * nslots + pg, wgt * dp[2]);
+            } else {
+              atomicAdd(&s_pgrad[pg], wgt * dp[0]);
+              atomicAdd(&s_pgrad[nslots + pg], wgt * dp[1]);
+              if (dp[2] != 0.f) atomicAdd(&s_pgrad[2 * nslots + pg], wgt * dp[2]);
+            }
           }
         } else {
           float E = 0.f, h;
@@ -711,7 +771,7 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
         }
       }
     }
-    if (MODE == kAdjoint) continue;
+    if (is_adjoint(MODE)) continue;
     // the gradient is -8 (gx, gy, gz); bl holds -8 x_local, and scaling by -8 is exact, so
     // fma(-8 x, g', acc) and fma(-8, g', acc) equal the products with the scaled gradient
     if (ENERGY) acc[0] += e;
@@ -815,7 +875,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       int voxel;
       unsigned in_any;
       bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
-      if (MODE != kAdjoint) bead_acc(acc, Eg, g, xl);
+      if (!is_adjoint(MODE)) bead_acc(acc, Eg, g, xl);
       st.n[0] += in_any;
       if (voxel >= 0) {
         st.n[1] += 1;
@@ -856,7 +916,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     phase_end(st, kPhScan);
     unsigned npair = 0;  // wave-wide count, credited to lane 0
     for (int q = 0; q < nposes; ++q) {
-      if (q == 1 && MODE != kAdjoint) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
+      if (q == 1 && !is_adjoint(MODE)) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
         uA = wave_sum_all(acc, lane);
 #pragma unroll
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
@@ -913,7 +973,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     st.n[3] += lane == 0 ? npair : 0u;
     wave_sync();
   }
-  if (MODE == kAdjoint) return;  // the parameter derivatives went to the wavefront's table
+  if (is_adjoint(MODE)) return;  // the parameter derivatives went to the wavefront's table
   const float u = wave_sum_all(acc, lane);
   if (nposes == 2)
     uB = u;
@@ -1289,7 +1349,7 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
 }
 
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
-__global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : kMaxWavesPerBlock))
+__global__ void __launch_bounds__(kWave * kMaxWavesPerBlock)
     pose_kernel(KArgs a) {
   extern __shared__ float4 smem[];
   const int nwaves = blockDim.x / kWave;
@@ -1308,12 +1368,14 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
   const int nrec_lds = REC_LDS ? a.nrec + 1 : 0;
   WaveScratch* s_ws = reinterpret_cast<WaveScratch*>(s_rec + nrec_lds);
   float* s_mat = reinterpret_cast<float*>(s_ws + nwaves);
-  // adjoint: one pair-table gradient [3][nslots] per wavefront
-  float* s_pgrad = MODE == kAdjoint ? s_mat + nwaves * (kBatch * 12) : nullptr;
-  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, s_kh, PLATEAU, EXACT, s_pgrad, nwaves * 3 * nslots);
+  // kAdjoint: the block's fixed-point pair-table gradient; kAdjointWave: one float table
+  // [3][nslots] per wavefront
+  float* s_pgrad = is_adjoint(MODE) ? s_mat + nwaves * (kBatch * 12) : nullptr;
+  stage_lds(a, REC_LDS ? s_rec : nullptr, s_par, s_side, s_kh, PLATEAU, EXACT, s_pgrad,
+            MODE == kAdjoint ? adjoint_table_floats(nslots) : nwaves * 3 * nslots);
   __syncthreads();
   WaveScratch* ws = s_ws + wave;
-  float* w_pgrad = MODE == kAdjoint ? s_pgrad + wave * 3 * nslots : nullptr;
+  float* w_pgrad = MODE == kAdjointWave ? s_pgrad + wave * 3 * nslots : s_pgrad;
   WaveStats st = {};
 #ifdef ATJ_PHASE_CLOCKS
   st.last = __builtin_amdgcn_s_memtime();
@@ -1332,8 +1394,8 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
     // g + 2G, ...), so each wavefront's table sums the same poses in the same order every run
     long long wave_id = (long long)blockIdx.x * nwaves + wave;
     const long long stride_claims = (long long)gridDim.x * nwaves;
-    for (Claim claim = MODE == kAdjoint ? claim_of(a, wave_id) : first_claim(a, lane); claim.k0 < a.N;
-         claim = MODE == kAdjoint ? claim_of(a, wave_id += stride_claims) : next_claim(a, lane)) {
+    for (Claim claim = is_adjoint(MODE) ? claim_of(a, wave_id) : first_claim(a, lane); claim.k0 < a.N;
+         claim = is_adjoint(MODE) ? claim_of(a, wave_id += stride_claims) : next_claim(a, lane)) {
       const int k0 = (int)claim.k0;
       const int nbatch = claim.n;
       int p_l = 0;
@@ -1341,7 +1403,7 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
       wave_sync();  // the previous claim's readers of Ms are done
       if (lane < nbatch) {
         p_l = a.pose_order ? a.pose_order[k0 + lane] : k0 + lane;
-        if (MODE == kAdjoint) w_l = a.weights[p_l];
+        if (is_adjoint(MODE)) w_l = a.weights[p_l];
         float Mp[16];
         load_pose_matrix(a.poses + (size_t)p_l * (qt ? 7 : 16), qt, Mp);
 #pragma unroll
@@ -1358,7 +1420,7 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
         P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
         P[0].nb = job.ligand.nbeads;
         P[0].M = Ms + 12 * b;
-        P[0].wgt = MODE == kAdjoint ? lane_value(w_l, b) : 0.f;
+        P[0].wgt = is_adjoint(MODE) ? lane_value(w_l, b) : 0.f;
         P[1] = P[0];
         P[1].nb = 0;
         p[1] = -1;
@@ -1370,13 +1432,13 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
             P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
             P[1].nb = job2.ligand.nbeads;
             P[1].M = Ms + 12 * (b + 1);
-            P[1].wgt = MODE == kAdjoint ? lane_value(w_l, b + 1) : 0.f;
+            P[1].wgt = is_adjoint(MODE) ? lane_value(w_l, b + 1) : 0.f;
           }
         }
         float u[2];
         pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, w_pgrad, ws, P[0], P[1], st,
                                                              lane, u[0], u[1]);
-        if (MODE != kAdjoint) {
+        if (!is_adjoint(MODE)) {
           if (a.flags & ATJ_GRAD_QT) {
             write_energrad_qt(a, p[0], u[0], lane);
             if (p[1] >= 0) write_energrad_qt(a, p[1], u[1], lane);
@@ -1402,16 +1464,21 @@ __global__ void __launch_bounds__(kWave *(MODE == kAdjoint ? kMaxAdjointWaves : 
       if (lane == 0) atomicAdd(a.stats + ATJ_NSTATS + i, st.clk[i]);
 #endif
   }
-  if (MODE == kAdjoint) {  // the block's row of partial sums: wavefront tables added in order
+  if (is_adjoint(MODE)) {  // the block's row of partial sums
     __syncthreads();
     float* out = a.partial + (size_t)blockIdx.x * 3 * ntp;
     const int stride = pair_stride(a.nlt);
     for (int i = threadIdx.x; i < 3 * ntp; i += blockDim.x) {
       const int c = i / ntp, k = i - c * ntp, rt = k / a.nlt;
       const int idx = c * nslots + rt * stride + (k - rt * a.nlt);
-      float v = s_pgrad[idx];
-      for (int w = 1; w < nwaves; ++w) v += s_pgrad[w * 3 * nslots + idx];
-      out[i] = v;
+      if (MODE == kAdjoint) {  // the fixed-point sum, plus the float sum of large terms
+        const unsigned long long* t = reinterpret_cast<const unsigned long long*>(s_pgrad);
+        out[i] = fixed_value(t + idx, 3 * nslots) + s_pgrad[2 * kFixWords * 3 * nslots + idx];
+      } else {  // the wavefront tables added in order
+        float v = s_pgrad[idx];
+        for (int w = 1; w < nwaves; ++w) v += s_pgrad[w * 3 * nslots + idx];
+        out[i] = v;
+      }
     }
   }
 }
@@ -1982,7 +2049,7 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   const bool auto_blocks = blocks == 0;
   if (auto_blocks) blocks = resident;
   if (a.batch == 0) a.batch = claim_size(a.N, (long long)min(blocks, resident) * wpb);
-  if (a.tail_batch == 0 || a.tail_batch >= a.batch || MODE == kAdjoint) a.tail_batch = a.batch;
+  if (a.tail_batch == 0 || a.tail_batch >= a.batch || is_adjoint(MODE)) a.tail_batch = a.batch;
   // a launch whose poses all fall in the tail (its rounds of claims cover them) is claimed
   // tail_batch at a time from the start
   const int tail_rounds = a.tail_start;
@@ -2004,7 +2071,7 @@ int launch_pose_kernel_t(KArgs& a, int wpb, size_t lds, int blocks, const DevInf
   } else {
     a.tail_start = a.N;
   }
-  if (MODE != kAdjoint) {  // the adjoint's claims are static: no work counter
+  if (!is_adjoint(MODE)) {  // the adjoint's claims are static: no work counter
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(1), 0, stream, a.counter, 0);
     HIPCHK(hipGetLastError());
   }
@@ -2025,22 +2092,25 @@ int launch_flags(KArgs& a, int wpb, size_t lds, int blocks, const DevInfo& d, hi
   // the adjoint's parameter derivatives are those of the energy: the gradient convention
   // (exact or force-shifted) does not enter them
   const bool plateau = (a.flags & ATJ_NB_PLATEAU) != 0;
-  const bool exact = MODE != kAdjoint && (a.flags & ATJ_GRAD_EXACT) != 0;
-  if (plateau && exact) return launch_layout<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
+  if constexpr (!is_adjoint(MODE)) {
+    if (a.flags & ATJ_GRAD_EXACT) {
+      if (plateau) return launch_layout<MODE, REC_LDS, true, true>(a, wpb, lds, blocks, d, stream);
+      return launch_layout<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
+    }
+  }
   if (plateau) return launch_layout<MODE, REC_LDS, true, false>(a, wpb, lds, blocks, d, stream);
-  if (exact) return launch_layout<MODE, REC_LDS, false, true>(a, wpb, lds, blocks, d, stream);
   return launch_layout<MODE, REC_LDS, false, false>(a, wpb, lds, blocks, d, stream);
 }
 
 template <int MODE>
 int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
-  if (!a.counter && MODE != kAdjoint) return fail(ATJ_ERR_INVALID, "work counter is NULL");
+  if (!a.counter && !is_adjoint(MODE)) return fail(ATJ_ERR_INVALID, "work counter is NULL");
   if (!a.jobs) return fail(ATJ_ERR_INVALID, "jobs is NULL");
   int wpb = launch && launch->waves_per_block > 0 ? launch->waves_per_block : 4;
   if (wpb > kMaxWavesPerBlock) return fail(ATJ_ERR_INVALID, "waves_per_block > 16");
-  if (MODE == kAdjoint) wpb = min(wpb, kMaxAdjointWaves);
   const uint32_t options = launch ? launch->options : 0u;
-  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_CLAIM_MASK | ATJ_LAUNCH_TAIL_MASK | ATJ_LAUNCH_TAIL_ROUNDS_MASK))
+  if (options & ~(ATJ_LAUNCH_REC_GLOBAL | ATJ_LAUNCH_CLAIM_MASK | ATJ_LAUNCH_TAIL_MASK | ATJ_LAUNCH_TAIL_ROUNDS_MASK |
+                  ATJ_LAUNCH_ADJOINT_WAVE_TABLES))
     return fail(ATJ_ERR_INVALID, "unknown launch option");
   const int claim = (int)((options & ATJ_LAUNCH_CLAIM_MASK) >> 4);
   if (claim != 0 && claim != 1 && claim != 2 && claim != 4 && claim != kBatch)
@@ -2056,16 +2126,22 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
   int rc = device_info(d);
   if (rc) return rc;
   const int nslots = pair_slots(a.nrt, a.nlt);
-  const size_t fixed = pair_lds_bytes(nslots);
+  const size_t fixed = pair_lds_bytes(nslots) + (MODE == kAdjoint ? adjoint_table_floats(nslots) * sizeof(float) : 0);
   const size_t per_wave = sizeof(WaveScratch) +
                           (MODE == kMinimize ? sizeof(BatchScratch) : kBatch * 12 * sizeof(float)) +
-                          (MODE == kAdjoint ? 3 * (size_t)nslots * sizeof(float) : 0);
+                          (MODE == kAdjointWave ? 3 * (size_t)nslots * sizeof(float) : 0);
+  if constexpr (MODE == kAdjoint) {
+    // the per-wavefront float tables when asked for, or when the block's fixed-point table
+    // leaves no room for a wavefront (pair tables of more than ~1,300 slots)
+    if ((options & ATJ_LAUNCH_ADJOINT_WAVE_TABLES) || fixed + per_wave > d.lds_per_block)
+      return launch_pose_kernel<kAdjointWave>(a, launch, stream);
+  }
   const size_t rec_bytes = (size_t)(a.nrec + 1) * sizeof(float4);
   bool rec_lds = !(options & ATJ_LAUNCH_REC_GLOBAL);
-  if (MODE == kAdjoint) {
-    // every wavefront holds its own gradient table: as many wavefronts as fit beside the pair
-    // table, and the receptor in LDS only when it costs no wavefront (C4: 8 wavefronts with
-    // the receptor read through L1 take 0.45 ms, 6 with it in LDS 0.48 ms)
+  if (is_adjoint(MODE)) {
+    // as many wavefronts as fit beside the pair table and the gradient tables, and the
+    // receptor in LDS only when it costs no wavefront (C4 with per-wavefront tables: 8
+    // wavefronts with the receptor read through L1 take 0.45 ms, 6 with it in LDS 0.48 ms)
     int w_lds = wpb, w_glob = wpb;
     while (w_lds > 1 && fixed + w_lds * per_wave + rec_bytes > d.lds_per_block) --w_lds;
     while (w_glob > 1 && fixed + w_glob * per_wave > d.lds_per_block) --w_glob;

@@ -170,8 +170,11 @@ typedef struct AtjPoseState {
 #define ATJ_LAUNCH_TAIL_ROUNDS(r) ((uint32_t)(r) << 12)  /* length of the tail: the last r rounds of
                                                            claims, r = 1..15 (bits 12-15; 0 = 1) */
 #define ATJ_LAUNCH_TAIL_ROUNDS_MASK 0xF000u
+#define ATJ_LAUNCH_ADJOINT_WAVE_TABLES 0x10000u  /* adjoint: per-wavefront float tables instead of the
+                                                   block's fixed-point table (the form used anyway
+                                                   when the fixed-point table does not fit in LDS) */
 typedef struct AtjLaunch {
-  int32_t waves_per_block;  /* 1..16 (0 = default 4); the adjoint uses at most 12 */
+  int32_t waves_per_block;  /* 1..16 (0 = default 4) */
   int32_t blocks;           /* 0 = as many as the device holds, fewer for small launches */
   uint32_t options;         /* ATJ_LAUNCH_* */
 } AtjLaunch;
@@ -259,9 +262,13 @@ int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float
  * (upstream dL/dE_pose) accumulates sum_pose w * dE_pose/d(rc, ac, emin) into
  * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated), recomputing
  * the pairs (nothing per pair is stored).  Reduce the slabs with atj_reduce_partials.
- * Requires blocks = launch->blocks > 0.  The sums are reproducible bit for bit: each
- * wavefront adds into a table of its own, takes its poses in a fixed order, and the tables
- * and slabs are added in a fixed order.  energy (nullable) and dEdM (nullable, needs energy)
+ * Requires blocks = launch->blocks > 0.  The sums are reproducible bit for bit for a given
+ * number of blocks: a block adds each pair's terms, rounded to multiples of 2^-48, into 64-bit
+ * fixed-point sums (exact, in any order, for terms below 2^48 in magnitude; larger terms,
+ * infinities and NaNs go to float sums), and the slabs are added in a fixed order.  Pair tables
+ * too large for the block's fixed-point table in LDS (or ATJ_LAUNCH_ADJOINT_WAVE_TABLES) take
+ * per-wavefront float tables: each wavefront takes its poses in a fixed order, and the tables
+ * are added in a fixed order (reproducible for a given launch shape).  energy (nullable) and dEdM (nullable, needs energy)
  * receive what atj_energrad writes, by an energrad launch before the gradient pass (using
  * counter); with both NULL only the gradient pass runs and counter may be NULL. */
 int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,

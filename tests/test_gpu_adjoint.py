@@ -164,3 +164,27 @@ def test_c4_param_gradient_vs_fd():
             assert abs(gr[a, b] - fd) <= 5e-4 * max(abs(fd), 1e-2 * np.abs(gr).max()), (a, b, gr[a, b], fd)
     print(f"C4 pair-table gradient vs FP64 FD (8 type pairs x (rmin, |emin|)): rel max {max(errs):.1e} median "
           f"{np.median(errs):.1e}")
+
+
+def test_adjoint_fixed_point_sums_do_not_depend_on_order(sysc1):
+    """The block's fixed-point sums are exact for terms below 2^48: with one block, the poses
+    spread over 1, 5 or 16 wavefronts (different orders of the adds) give the same bits.  The
+    per-wavefront float tables (the form for pair tables too large for the fixed-point one)
+    agree to float rounding."""
+    from attract_jax_amd.autodiff import param_adjoint
+    w, system, ligs, grid = sysc1
+    mat = mat_from_qt(w.quat, w.trans)
+    mat[:, 3, :3] *= 0.55
+    M = torch.tensor(mat, dtype=torch.float32, device=DEV).reshape(-1, 16)
+    rng = np.random.default_rng(5)
+    wts = torch.tensor(rng.uniform(-1.0, 1.5, len(mat)), dtype=torch.float32, device=DEV)
+    ff = system.ff
+    one = [param_adjoint(system, ligs[0], M, ff, wts, EXACT, blocks=1, waves_per_block=n) for n in (1, 5, 16)]
+    assert one[0].abs().max() > 0
+    assert torch.equal(one[0], one[1]) and torch.equal(one[0], one[2])
+    fixed = param_adjoint(system, ligs[0], M, ff, wts, EXACT, blocks=64, waves_per_block=16)
+    waves = param_adjoint(system, ligs[0], M, ff, wts, EXACT, blocks=64, waves_per_block=16,
+                          options=_lib.ATJ_LAUNCH_ADJOINT_WAVE_TABLES)
+    scale = one[0].abs().max()
+    assert ((fixed - one[0]).abs().max() / scale).item() < 1e-6
+    assert ((waves - one[0]).abs().max() / scale).item() < 1e-5
