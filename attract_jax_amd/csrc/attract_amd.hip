@@ -482,49 +482,37 @@ __device__ __forceinline__ void pair_param_grad(float dsq, float u, const float4
   d[2] = lo && neg ? -2.f : 0.f;  // ivor - 1
 }
 
-// Fixed-point sums of the pair-table gradient.  A sum is kFixWords signed 64-bit words w[c]
-// (c = 0, 1, 2), worth sum_c w[c] 2^(32c - kFixFrac).  A term is rounded to a multiple of
-// 2^-kFixFrac; its 24-bit significand, shifted to its place, straddles at most two words, and
-// each word takes a signed addend below 2^32, so 2^31 adds per word cannot overflow.  Integer
-// adds give the same sum in any order, and LDS integer atomics are much faster than float ones
-// (ds_add_u64 ~17x the rate of ds_add_f32 on gfx950).
-constexpr int kFixWords = 3;
-constexpr int kFixFrac = 48;  // resolution 2^-48; terms below 2^48 in magnitude are summed exactly
+// Fixed-point sums of the pair-table gradient.  A term v is rounded to a multiple of 2^-43:
+// with a = v 2^12 (exact), B = rne(a) and r = a - B (exact, |r| <= 1/2), the term is
+// B 2^-12 + L 2^-43 with L = rne(r 2^31), |L| <= 2^30.  A sum is two signed 64-bit words,
+// w1 = sum of B and w0 = sum of L, worth w1 2^-12 + w0 2^-43.  Both take sign-extended 32-bit
+// addends, so 2^31 adds cannot overflow them.  Integer adds give the same sum in any order,
+// and LDS integer atomics are much faster than float ones (ds_add_u64 ~17x the rate of
+// ds_add_f32 on gfx950).  Terms of 2^19 and above in magnitude (B beyond 32 bits), infinities
+// and NaNs go to a float sum instead, the one part whose order is not fixed.
+constexpr int kFixWords = 2;
 
-// Adds v to the fixed-point sum whose words are t[0], t[stride], t[2 stride].  Terms of 2^48
-// and above, infinities and NaNs are added to the float *big instead.
+// Adds v to the fixed-point sum whose words are t[0] (w0) and t[stride] (w1), or to the float
+// *big for terms beyond the fixed-point range.
 __device__ __forceinline__ void fixed_add(unsigned long long* t, unsigned stride, float* big, float v) {
-  const unsigned bits = __float_as_uint(v);
-  // v = +-m 2^(ex - 150) (zero and subnormals too: they round to 0 below); p is the place of
-  // m's lowest bit in units of 2^-kFixFrac
-  const int p = (int)__builtin_amdgcn_ubfe(bits, 23, 8) + (kFixFrac - 150);
-  if (p > 32 * kFixWords - 24) {
+  const float a = v * 0x1p12f;
+  if (!(fabsf(a) < 0x1p31f)) {  // also NaN
     atomicAdd(big, v);
     return;
   }
-  // below the resolution: round the magnitude to the nearest unit (halves away from zero); a
-  // shift of 25 or more leaves 0, as m < 2^24
-  const unsigned s = (unsigned)min(max(-p, 0), 31);
-  const unsigned m = (((bits & 0x7fffffu) | 0x800000u) + ((1u << s) >> 1)) >> s;
-  const unsigned q = (unsigned)max(p, 0);
-  const unsigned long long x = (unsigned long long)m << (q & 31u);
-  const unsigned neg = (unsigned)((int)bits >> 31);  // all ones for v < 0: two's complement negation
-  unsigned long long* w = t + __umul24(q >> 5, stride);
-  const unsigned long long neg64 = (unsigned long long)(long long)(int)neg;
-  atomicAdd(w, ((x & 0xffffffffull) ^ neg64) - neg64);
-  const unsigned hi = (unsigned)(x >> 32);  // below 2^25
-  if (hi) atomicAdd(w + stride, (unsigned long long)(long long)(int)((hi ^ neg) - neg));
+  const float b = __builtin_rintf(a);
+  const int lo = (int)__builtin_rintf((a - b) * 0x1p31f);
+  const int hi = (int)b;
+  atomicAdd(t, (unsigned long long)(long long)lo);
+  atomicAdd(t + stride, (unsigned long long)(long long)hi);
 }
 
-// The value of a fixed-point sum, rounded to float: carries are propagated so that the two
-// lower words lie in [0, 2^32), then the parts are added in double from the bottom up.
+// The value of a fixed-point sum, rounded to float: w0's carries go into w1, leaving w0 in
+// [0, 2^31), then the two parts are added in double.
 __device__ __forceinline__ float fixed_value(const unsigned long long* t, unsigned stride) {
-  long long w0 = (long long)t[0], w1 = (long long)t[stride], w2 = (long long)t[2 * stride];
-  w1 += w0 >> 32;
-  w2 += w1 >> 32;
-  const double r0 = (double)(w0 & 0xffffffffll), r1 = (double)(w1 & 0xffffffffll);
-  const double low = fma(r1, 0x1p-16, r0 * 0x1p-48);
-  return (float)fma((double)w2, 0x1p16, low);
+  const long long w0 = (long long)t[0];
+  const long long w1 = (long long)t[stride] + (w0 >> 31);
+  return (float)fma((double)w1, 0x1p-12, (double)(w0 & 0x7fffffffll) * 0x1p-43);
 }
 
 // LDS pair-table rows (one per receptor type) have an odd stride of nlt | 1 records, so
@@ -2132,7 +2120,7 @@ int launch_pose_kernel(KArgs& a, const AtjLaunch* launch, hipStream_t stream) {
                           (MODE == kAdjointWave ? 3 * (size_t)nslots * sizeof(float) : 0);
   if constexpr (MODE == kAdjoint) {
     // the per-wavefront float tables when asked for, or when the block's fixed-point table
-    // leaves no room for a wavefront (pair tables of more than ~1,300 slots)
+    // leaves no room for a wavefront (pair tables of more than ~1,800 slots)
     if ((options & ATJ_LAUNCH_ADJOINT_WAVE_TABLES) || fixed + per_wave > d.lds_per_block)
       return launch_pose_kernel<kAdjointWave>(a, launch, stream);
   }

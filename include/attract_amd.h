@@ -263,8 +263,8 @@ int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float
  * partial[blocks][3][nrt*nlt] (one slab per block, written, not accumulated), recomputing
  * the pairs (nothing per pair is stored).  Reduce the slabs with atj_reduce_partials.
  * Requires blocks = launch->blocks > 0.  The sums are reproducible bit for bit for a given
- * number of blocks: a block adds each pair's terms, rounded to multiples of 2^-48, into 64-bit
- * fixed-point sums (exact, in any order, for terms below 2^48 in magnitude; larger terms,
+ * number of blocks: a block adds each pair's terms, rounded to multiples of 2^-43, into two-word
+ * 64-bit fixed-point sums (exact, in any order, for terms below 2^19 in magnitude; larger terms,
  * infinities and NaNs go to float sums), and the slabs are added in a fixed order.  Pair tables
  * too large for the block's fixed-point table in LDS (or ATJ_LAUNCH_ADJOINT_WAVE_TABLES) take
  * per-wavefront float tables: each wavefront takes its poses in a fixed order, and the tables

@@ -167,7 +167,7 @@ def test_c4_param_gradient_vs_fd():
 
 
 def test_adjoint_fixed_point_sums_do_not_depend_on_order(sysc1):
-    """The block's fixed-point sums are exact for terms below 2^48: with one block, the poses
+    """The block's fixed-point sums are exact for terms below 2^19: with one block, the poses
     spread over 1, 5 or 16 wavefronts (different orders of the adds) give the same bits.  The
     per-wavefront float tables (the form for pair tables too large for the fixed-point one)
     agree to float rounding."""
