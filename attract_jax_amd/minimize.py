@@ -3,7 +3,8 @@
 `minimize` packs the poses of any number of ligands (uneven bead counts) into ONE
 persistent launch that runs every step (atj_minimize).  `minimize_plain` is the
 per-job comparison form: one ligand at a time, one energrad launch plus one
-optimizer launch per step.
+optimizer launch per step.  `PlainGraph` captures the per-job form's steps once in a
+HIP graph and replays them, so the per-step launches cost no host work.
 
 Optimizer semantics, per pose and per parameter (quaternion x, y, z, w and
 translation): optax.adam defaults (test4.py:446-453): m = (1-b1) g + b1 m,
@@ -132,3 +133,37 @@ def minimize_plain(system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, b
                                     _ptr(system.counter), ctypes.c_void_p(0), st))
         _lib.check(lib.atj_optimizer_step(_ptr(dEdM), _ptr(state.energy), N, ctypes.byref(ss), ctypes.byref(opt), st))
     return state
+
+
+class PlainGraph:
+    """The per-job form's launches (one energrad and one optimizer launch per step, as
+    minimize_plain issues them) captured once in a HIP graph (torch.cuda.CUDAGraph over the
+    C-ABI calls) and replayed on the current stream.  A replay runs the `nsteps` steps on the
+    arrays of `state` as they are at the time of the replay: reset them (state.pose.copy_(...),
+    m, v, count zeroed) to start again.  The captured launches are those minimize_plain makes,
+    so a replay gives the same bits as minimize_plain from the same state.  Every launch resets
+    the system's work counter with a fill kernel; nothing in the graph is a memset or a copy.
+    The graph holds the system's counter: replays of graphs of one system must not overlap."""
+
+    def __init__(self, system, ligand, state, nsteps, lr_rot, lr_trans, beta1=0.9, beta2=0.999, eps=1e-8,
+                 flags=None, jobs=None, *, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0, launch=None):
+        d = system.device
+        if jobs is None:
+            jobs = system.jobs_tensor([ligand])  # uploaded before the capture: no copy in the graph
+        self.jobs, self.state, self.nsteps = jobs, state, int(nsteps)
+        kw = dict(flags=flags, jobs=jobs, optimizer=optimizer, eps_root=eps_root, gravity=gravity, launch=launch)
+        args = (lr_rot, lr_trans, beta1, beta2, eps)
+        # one step on copies of the state first, so that every kernel is loaded and every launch
+        # attribute set before the capture
+        tmp = PoseState(state.pose.clone(), state.m.clone(), state.v.clone(), state.count.clone(),
+                        state.energy.clone())
+        minimize_plain(system, ligand, tmp, 1, *args, **kw)
+        torch.cuda.synchronize(d)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, capture_error_mode="relaxed"):
+            minimize_plain(system, ligand, state, nsteps, *args, **kw)
+        torch.cuda.synchronize(d)
+
+    def replay(self):
+        self.graph.replay()
+        return self.state
