@@ -242,7 +242,7 @@ __device__ __forceinline__ float div_rn(float a, float b, float y) {
 // raw = sum_c w_c G_c (4 channels); dEdv = d(sum_c w_c E_c)/dv (exact mode).
 // Cell and point indices within one type's slab are 32-bit; one 64-bit multiply-add
 // per bead places the slab.  Requires 0 <= v < dim - 1 on every axis.
-template <int LAYOUT>
+template <int LAYOUT, bool ENERGY = true>
 __device__ __forceinline__ void trilinear(const float* g, unsigned slab, int t, int dx, int dy, int dz, float vx,
                                           float vy, float vz, bool exact, float raw[4], float dEdv[3]) {
   const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
@@ -270,7 +270,7 @@ __device__ __forceinline__ void trilinear(const float* g, unsigned slab, int t, 
   for (int i = 0; i < 8; ++i) {
     const int ax = i >> 2, ay = (i >> 1) & 1, az = i & 1;
     const float w = (wx[ax] * wy[ay]) * wz[az];
-    raw[0] = fmaf(w, c[i].x, raw[0]);
+    if (ENERGY || exact) raw[0] = fmaf(w, c[i].x, raw[0]);
     raw[1] = fmaf(w, c[i].y, raw[1]);
     raw[2] = fmaf(w, c[i].z, raw[2]);
     raw[3] = fmaf(w, c[i].w, raw[3]);
@@ -289,7 +289,9 @@ __device__ __forceinline__ void trilinear(const float* g, unsigned slab, int t, 
 // list, or with shell-ordered lists the leading entries of shell
 // min(floor(distance from the voxel's grid point / shell width), nshell - 1).  The list
 // entry is loaded before the grid corners so that both loads are in flight together.
-template <int LAYOUT>
+// Without ENERGY (minimizer steps whose energy is not kept) E is left 0 in non-exact mode:
+// the energy channel is not interpolated, the gradient is the same.
+template <int LAYOUT, bool ENERGY = true>
 __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const float xp[3], bool do_grid,
                                           bool do_lists, bool exact, float& E, float g[3], int& voxel, int& cnt,
                                           int& start, unsigned& in_any, float raw_out[4]) {
@@ -335,7 +337,7 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
   float raw[4], dEdv[3];
   float inv = 0.f;
   if (inside) {
-    trilinear<LAYOUT>(G.inner, G.slab, gtype, G.dx, G.dy, G.dz, vx, vy, vz, exact, raw, dEdv);
+    trilinear<LAYOUT, ENERGY>(G.inner, G.slab, gtype, G.dx, G.dy, G.dz, vx, vy, vz, exact, raw, dEdv);
     if (exact) inv = 1.f / G.spacing;
     in_any = 1;
   } else if (G.outer != nullptr) {
@@ -343,7 +345,7 @@ __device__ __forceinline__ void bead_grid(const GridView& G, int gtype, const fl
     const bool in_outer =
         (ox >= 0.f) && (ox < G.lx2) && (oy >= 0.f) && (oy < G.ly2) && (oz >= 0.f) && (oz < G.lz2);
     if (!in_outer) return;
-    trilinear<LAYOUT>(G.outer, G.slab2, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
+    trilinear<LAYOUT, ENERGY>(G.outer, G.slab2, gtype, G.dx2, G.dy2, G.dz2, ox, oy, oz, exact, raw, dEdv);
     if (exact) inv = 0.5f / G.spacing;
     in_any = 1;
   } else {
@@ -647,7 +649,11 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
   const unsigned nslots = (unsigned)pair_slots(a.nrt, a.nlt);
   const unsigned zero_rec = 16u * nslots;  // byte offset of the all-zero record
   int carry = 0;
-  for (int base = 0; base < total; base += kWave) {
+  static_assert(S == 4 || S == 8, "segments of 4 or 8 list entries");
+  // bead of this lane's item in the window at `base`: the lanes whose first segment falls in
+  // the window mark it, and a prefix maximum hands every lane its bead (the bead of the
+  // window's first item is carried over from the previous window)
+  auto window_bead = [&](int base) {
     ws->mark[lane] = -1;
     wave_sync();
     const int d = excl - base;
@@ -655,29 +661,47 @@ __device__ __forceinline__ void pair_items(const KArgs& a, const float4* __restr
     wave_sync();
     const int jb = max(wave_max_scan(ws->mark[lane]), carry);
     carry = __builtin_amdgcn_readlane(jb, kWave - 1);
+    return jb;
+  };
+  // the list entries of this lane's segment in the window at `base`; inactive lanes read the
+  // all-padding segment at the head of the list array
+  auto load_words = [&](int base, int jb, unsigned w[S / 2]) {
     const int item = base + lane;
-    const bool act = item < total;
+    const int off = item < total ? __float_as_int(ws->xl[jb].w) + item * S : 0;
+    if (S == 8) {
+      const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);
+      w[0] = ent.x;
+      w[1] = ent.y;
+      w[S / 2 > 2 ? 2 : 0] = ent.z;
+      w[S / 2 > 3 ? 3 : 0] = ent.w;
+    } else {
+      const uint2 ent = *reinterpret_cast<const uint2*>(G.nblist + off);
+      w[0] = ent.x;
+      w[1] = ent.y;
+    }
+  };
+  // the next window's list entries are loaded while this window's pairs are evaluated
+  unsigned next[S / 2];
+  int jb_next = 0;
+  if (total > 0) {
+    jb_next = window_bead(0);
+    load_words(0, jb_next, next);
+  }
+  for (int base = 0; base < total; base += kWave) {
+    const int jb = jb_next;
+    unsigned words[S / 2];
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i) words[i] = next[i];
+    if (base + kWave < total) {
+      jb_next = window_bead(base + kWave);
+      load_words(base + kWave, jb_next, next);
+    }
     const float4 bx = ws->xp[jb];
     const float4 bl = ws->xl[jb];
     const unsigned col = (unsigned)__float_as_int(bx.w);  // 16 x ligand ff type
     const unsigned zrow = zero_rec - col;                 // the "row" whose column is the zero record
-    // inactive lanes read the all-padding segment at the head of the list array
-    const int off = act ? __float_as_int(bl.w) + item * S : 0;
-    static_assert(S == 4 || S == 8, "segments of 4 or 8 list entries");
     // -0 is the identity of float addition, so the first pair needs no add
     float e = -0.f, gx = -0.f, gy = -0.f, gz = -0.f;
-    unsigned words[S / 2];
-    if (S == 8) {
-      const uint4 ent = *reinterpret_cast<const uint4*>(G.nblist + off);
-      words[0] = ent.x;
-      words[1] = ent.y;
-      words[S / 2 > 2 ? 2 : 0] = ent.z;
-      words[S / 2 > 3 ? 3 : 0] = ent.w;
-    } else {
-      const uint2 ent = *reinterpret_cast<const uint2*>(G.nblist + off);
-      words[0] = ent.x;
-      words[1] = ent.y;
-    }
 #pragma unroll
     for (int h = 0; h < S / 4; ++h) {
       float4 rb[4];
@@ -862,7 +886,10 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       transform(M, xl[0], xl[1], xl[2], xp);
       int voxel;
       unsigned in_any;
-      bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
+      if (MODE == kMinimize && !energy)  // the grid energy channel is not interpolated
+        bead_grid<LAYOUT, false>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
+      else
+        bead_grid<LAYOUT>(G, gt, xp, do_grid, do_nb, EXACT, Eg, g, voxel, cnt, start, in_any, raw);
       if (!is_adjoint(MODE)) bead_acc(acc, Eg, g, xl);
       st.n[0] += in_any;
       if (voxel >= 0) {
