@@ -150,7 +150,9 @@ class PlainGraph:
         d = system.device
         if jobs is None:
             jobs = system.jobs_tensor([ligand])  # uploaded before the capture: no copy in the graph
-        self.jobs, self.state, self.nsteps = jobs, state, int(nsteps)
+        # the graph's launches hold device pointers of the system (grid, receptor, pair table,
+        # counter), the ligand, the job table and the state: they are kept alive with it
+        self.system, self.ligand, self.jobs, self.state, self.nsteps = system, ligand, jobs, state, int(nsteps)
         kw = dict(flags=flags, jobs=jobs, optimizer=optimizer, eps_root=eps_root, gravity=gravity, launch=launch)
         args = (lr_rot, lr_trans, beta1, beta2, eps)
         # one step on copies of the state first, so that every kernel is loaded and every launch
