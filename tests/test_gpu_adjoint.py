@@ -188,3 +188,27 @@ def test_adjoint_fixed_point_sums_do_not_depend_on_order(sysc1):
     scale = one[0].abs().max()
     assert ((fixed - one[0]).abs().max() / scale).item() < 1e-6
     assert ((waves - one[0]).abs().max() / scale).item() < 1e-5
+
+
+def test_adjoint_terms_beyond_the_fixed_point_range(sysc1):
+    """Terms of 2^19 and above go to the float sums beside the fixed-point words: with the pose
+    weights scaled by 2^24 (every in-plateau term of magnitude >= 2^-5 then leaves the
+    fixed-point range) the sums scale by 2^24 to float rounding, and the float sums agree with
+    the per-wavefront float tables."""
+    from attract_jax_amd.autodiff import param_adjoint
+    w, system, ligs, grid = sysc1
+    mat = mat_from_qt(w.quat, w.trans)
+    mat[:, 3, :3] *= 0.55
+    M = torch.tensor(mat, dtype=torch.float32, device=DEV).reshape(-1, 16)
+    rng = np.random.default_rng(7)
+    wts = torch.tensor(rng.uniform(0.5, 1.5, len(mat)), dtype=torch.float32, device=DEV)
+    ff = system.ff
+    base = param_adjoint(system, ligs[0], M, ff, wts, EXACT, blocks=64, waves_per_block=16)
+    big = param_adjoint(system, ligs[0], M, ff, wts * 2.0 ** 24, EXACT, blocks=64, waves_per_block=16)
+    waves = param_adjoint(system, ligs[0], M, ff, wts * 2.0 ** 24, EXACT, blocks=64, waves_per_block=16,
+                          options=_lib.ATJ_LAUNCH_ADJOINT_WAVE_TABLES)
+    scale = big.abs().max()
+    assert scale > 2.0 ** 19  # the float path was taken
+    assert torch.isfinite(big).all()
+    assert ((big - base * 2.0 ** 24).abs().max() / scale).item() < 1e-5
+    assert ((big - waves).abs().max() / scale).item() < 1e-5
