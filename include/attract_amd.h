@@ -181,10 +181,11 @@ typedef struct AtjLaunch {
 /* Poses are claimed 1, 2, 4 or 8 at a time (chosen from N and the wavefronts launched, or set
  * by ATJ_LAUNCH_CLAIM).  With ATJ_LAUNCH_TAIL the poses of the last rounds of claims (one round:
  * a claim for every wavefront the device holds at once) are claimed in smaller claims, which
- * even out the end of a persistent launch while the rest keeps the larger batches.  The first
- * claim of each wavefront in the blocks the device holds at once is fixed by its index,
- * further claims come from the work counter.  A pose's result does not depend on any of
- * these options. */
+ * even out the end of a persistent launch while the rest keeps the larger batches.  In
+ * atj_minimize the first claim of each wavefront in the blocks the device holds at once is
+ * fixed by its index, further claims come from the work counter.  atj_energrad and
+ * atj_energrad_adjoint take their claims in a fixed order (wavefront g of G: claims g, g + G,
+ * ...) and use no counter.  A pose's result does not depend on any of these options. */
 
 const char* atj_last_error(void);
 const char* atj_version(void);
@@ -195,7 +196,7 @@ const char* atj_version(void);
  * jobs/njobs give the ligand of each pose range; pose_order (nullable) is the
  * processing order of pose slots (a permutation of [0, N)).
  * energy: [N]; dEdM: [N][16] = dE/dM (nullable; with ATJ_GRAD_QT dE/d(q, t), see the flag);
- * counter: device int32 work counter (caller-provided workspace, reset by the call); stats:
+ * counter: unused (claims follow a fixed order; nullable, kept for the signature); stats:
  * nullable. */
 int atj_energrad(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
                  const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
@@ -269,8 +270,8 @@ int atj_ligand_rmsd(const float* poses, int32_t N, const float* lig, const float
  * too large for the block's fixed-point table in LDS (or ATJ_LAUNCH_ADJOINT_WAVE_TABLES) take
  * per-wavefront float tables: each wavefront takes its poses in a fixed order, and the tables
  * are added in a fixed order (reproducible for a given launch shape).  energy (nullable) and dEdM (nullable, needs energy)
- * receive what atj_energrad writes, by an energrad launch before the gradient pass (using
- * counter); with both NULL only the gradient pass runs and counter may be NULL. */
+ * receive what atj_energrad writes, by an energrad launch before the gradient pass; counter is
+ * unused and may be NULL. */
 int atj_energrad_adjoint(const AtjGrid* grid, const AtjForceField* ff, const AtjReceptor* rec,
                          const AtjJob* jobs_dev, int32_t njobs, const int32_t* pose_job,
                          const float* poses, int32_t N, const float* weights, float* energy,
