@@ -155,6 +155,7 @@ class ScoringSystem:
         self._rs = self.receptor.struct()
         self._alive = (g.inner, g.outer, g.nbgrid, g.nblist, None if whole else g.nbshell, self.receptor.beads,
                        self.ff)
+        self._last_jobs = None  # (ligand, its beads' address and count, its job table) of the last energrad
 
     def jobs_tensor(self, ligands, nsteps=0, lr_rot=1e-3, lr_trans=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
                     pose_counts=None, optimizer=_lib.ATJ_OPT_ADAM, eps_root=0.0, gravity=0.0):
@@ -187,8 +188,11 @@ class ScoringSystem:
         P = P.reshape(N, 7 if qt else 16)
         if grad_qt and not qt:
             raise ValueError("grad_qt needs quaternion poses (qt=True)")
-        if jobs is None:
-            jobs = self.jobs_tensor([ligand])
+        if jobs is None:  # the job table of the last ligand is kept: repeated calls upload nothing
+            key = (ligand.beads.data_ptr(), ligand.nbeads)
+            if self._last_jobs is None or self._last_jobs[0] is not ligand or self._last_jobs[1] != key:
+                self._last_jobs = (ligand, key, self.jobs_tensor([ligand]))
+            jobs = self._last_jobs[2]
             njobs = 1
         energy = torch.empty(N, dtype=torch.float32, device=d)
         dEdM = torch.empty((N, 4, 4), dtype=torch.float32, device=d) if grad else None
