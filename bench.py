@@ -346,22 +346,33 @@ def main():
         sub = BatchPlan(system, ligs, counts, w.nsteps, w.lr_rot, w.lr_trans)
         sst = PoseState.create(w.quat[idx], w.trans[idx], dev)
         sinit = sst.pose.clone()
-        ms = []
-        for _ in range(4):
-            sst.pose.copy_(sinit)
-            sst.m.zero_()
-            sst.v.zero_()
-            sst.count.zero_()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            minimize(system, sub, sst, w.nsteps)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms.append(e0.elapsed_time(e1))
-        share_ms = min(ms[1:])
+        # with 8 poses per wavefront, smaller claims even out ligands of different sizes: the
+        # full workload's claim options and two with smaller claims, the fastest kept
+        full_launch = system.launch
+        C, T = _lib.ATJ_LAUNCH_CLAIM, _lib.ATJ_LAUNCH_TAIL
+        keep = full_launch.options & _lib.ATJ_LAUNCH_REC_GLOBAL
+        share_ms, share_opt = float("inf"), None
+        for opt in (full_launch.options, keep | C(2), keep | C(8) | T(2, 2)):
+            system.launch = _lib.AtjLaunch(full_launch.waves_per_block, full_launch.blocks, opt)
+            ms = []
+            for _ in range(4):
+                sst.pose.copy_(sinit)
+                sst.m.zero_()
+                sst.v.zero_()
+                sst.count.zero_()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                minimize(system, sub, sst, w.nsteps)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            if min(ms[1:]) < share_ms:
+                share_ms, share_opt = min(ms[1:]), opt
+        system.launch = full_launch
         result["one_gpu_share_of_8"] = {
             "poses": int(sub.N), "kernel_ms": share_ms, "pose_steps_per_s": sub.N * w.nsteps / (share_ms * 1e-3),
-            "vs_full_rate": (sub.N * w.nsteps / (share_ms * 1e-3)) / (pose_steps / (k_ms * 1e-3))}
+            "vs_full_rate": (sub.N * w.nsteps / (share_ms * 1e-3)) / (pose_steps / (k_ms * 1e-3)),
+            "launch_options": share_opt}
         log(f"[bench] one GPU's share of eight ({sub.N} poses): {share_ms:.2f} ms, "
             f"{result['one_gpu_share_of_8']['vs_full_rate']:.2f} of the full-size rate")
 
