@@ -10,7 +10,9 @@ the scheduler
     launch (atj_minimize) advances all of them together;
   * runs each receptor group in chunks of `chunk_steps` ADAM steps on its own HIP
     stream, so groups overlap on the device and newly admitted jobs join the next
-    chunk of their group (no waiting for the running jobs to finish);
+    chunk of their group (no waiting for the running jobs to finish); with `chunk_work`,
+    a launch of few poses takes more steps, up to chunk_work pose-steps, so that a small
+    job is not stretched over many short launches;
   * advances, at each chunk boundary, the jobs with the least remaining work first
     (pose-steps times ligand beads), as many as fill the GPU (`run_poses` poses in
     flight): a short request overtakes long ones instead of sharing the device with
@@ -134,10 +136,11 @@ class Group:
 
 class Scheduler:
     def __init__(self, backend, hbm_budget, chunk_steps=25, initial_slots=1 << 16, clock=time.perf_counter,
-                 run_poses=1 << 16, max_wait=0.5):
+                 run_poses=1 << 16, max_wait=0.5, chunk_work=0):
         self.backend = backend
         self.budget = int(hbm_budget)
         self.chunk_steps = int(chunk_steps)
+        self.chunk_work = int(chunk_work)
         self.run_poses = int(run_poses)
         self.max_wait = float(max_wait)
         self.initial_slots = int(initial_slots)
@@ -295,7 +298,10 @@ class Scheduler:
             jobs = [j for j in g.jobs if j.id in chosen]
             if not jobs:
                 continue
-            steps = min(self.chunk_steps, max(j.remaining for j in jobs))
+            steps = self.chunk_steps
+            if self.chunk_work:  # launches of few poses take more steps, up to chunk_work pose-steps
+                steps = max(steps, -(-self.chunk_work // sum(j.npose for j in jobs)))
+            steps = min(steps, max(j.remaining for j in jobs))
             dirty = g.dirty or [j.id for j in jobs] != [j.id for j in g.chunk_jobs]
             g.chunk_jobs = jobs
             g.chunk_steps = steps
@@ -371,6 +377,8 @@ class GpuBackend:
         # group streams there are.
         self._free_streams = []
         self._build_stream = torch.cuda.Stream(self.device)
+        # launches of fewer poses than 8 per wavefront the device holds take their own claim size
+        self._small_launch = 8 * waves_per_block * torch.cuda.get_device_properties(self.device).multi_processor_count
         self._built = {}
         self.build_seconds = []       # time from each grid build's start to its grid being ready
         self.layouts = []             # layout of each group made
@@ -559,10 +567,17 @@ class GpuBackend:
             p = h["plan"]
             system = h["system"]
             lib = _lib.load()
+            launch = system.launch
+            if p["N"] < self._small_launch:
+                # fewer than 8 poses per wavefront of the device: the launch chooses its own claim
+                # size (down to one pose per claim, so that the poses spread over every CU)
+                keep = launch.options & ~(_lib.ATJ_LAUNCH_CLAIM_MASK | _lib.ATJ_LAUNCH_TAIL_MASK |
+                                          _lib.ATJ_LAUNCH_TAIL_ROUNDS_MASK)
+                launch = _lib.AtjLaunch(launch.waves_per_block, launch.blocks, keep)
             _lib.check(lib.atj_minimize(ctypes.byref(system._gs), ctypes.byref(system._fs),
                                         ctypes.byref(system._rs), _ptr(p["jobs"]), p["njobs"], _ptr(p["pose_job"]),
                                         _ptr(p["order"]), p["N"], ctypes.byref(h["state"].struct()), int(steps),
-                                        system.flags, ctypes.byref(system.launch), _ptr(h["counter"]),
+                                        system.flags, ctypes.byref(launch), _ptr(h["counter"]),
                                         ctypes.c_void_p(0), ctypes.c_void_p(h["stream"].cuda_stream)))
             ev = torch.cuda.Event()
             ev.record(h["stream"])

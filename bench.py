@@ -504,7 +504,7 @@ def main():
                         launch_options=system.launch.options & (_lib.ATJ_LAUNCH_CLAIM_MASK | _lib.ATJ_LAUNCH_TAIL_MASK |
                                                                     _lib.ATJ_LAUNCH_TAIL_ROUNDS_MASK))
         budget = int(args.serve_budget_gib * 2 ** 30)
-        sch = Scheduler(be, hbm_budget=budget, chunk_steps=25)
+        sch = Scheduler(be, hbm_budget=budget, chunk_steps=25, chunk_work=1_000_000)
         # host time of each scheduler tick (admission builds grids on the host thread) and the
         # caching allocator's retries (a retry frees every cached block, synchronizing the device)
         ticks, tick = [], sch.tick
@@ -557,7 +557,8 @@ def main():
                          for k, (n, t, m) in host_ops.items()},
             "plain_grid_gb": [round(min(plain_gb), 3), round(max(plain_gb), 3)],
             "makespan_s": makespan, "pose_steps": ps, "aggregate_pose_steps_per_s": ps / makespan,
-            "poses_per_job": [int(min(nposes)), int(max(nposes))], "chunk_steps": sch.chunk_steps}
+            "poses_per_job": [int(min(nposes)), int(max(nposes))], "chunk_steps": sch.chunk_steps,
+            "chunk_work": sch.chunk_work}
         log(f"[bench] serving: p50 {result['serving']['p50_latency_s']:.3f}s p99 "
             f"{result['serving']['p99_latency_s']:.3f}s, admission wait p99 "
             f"{result['serving']['p99_admission_wait_s']:.3f}s, {sch.evictions} evictions, {corner} corner / "

@@ -364,3 +364,19 @@ def test_groups_launch_once_their_grid_is_built():
     assert "bad" in be.dropped and "bad" not in s.groups
     assert sorted(j.spec.receptor_key for j in s.done if j.state == "done") == ["a", "b"]
     assert be.launches[0][0] == "a"
+
+
+def test_chunk_work_runs_small_launches_longer():
+    """With chunk_work, a launch of few poses takes more steps (up to chunk_work pose-steps, at
+    least chunk_steps, at most what its jobs have left); a launch of many poses keeps chunk_steps."""
+    clock = FakeClock()
+    be = FakeBackend(clock, {"r": 100, "s": 100})
+    s = Scheduler(be, hbm_budget=10 ** 9, chunk_steps=10, initial_slots=16, clock=clock, chunk_work=2000)
+    s.submit(spec("r", 20, 150))    # 20 poses: 100 steps per launch (2000 / 20), then the last 50
+    s.submit(spec("s", 400, 30))    # 400 poses: 2000 / 400 = 5 < chunk_steps, so 10 steps per launch
+    drive(s, clock)
+    steps_r = [st for key, _, st in be.launches if key == "r"]
+    steps_s = [st for key, _, st in be.launches if key == "s"]
+    assert steps_r == [100, 50]
+    assert steps_s == [10, 10, 10]
+    assert all(j.state == "done" and j.steps_done == j.spec.nsteps for j in s.done)
