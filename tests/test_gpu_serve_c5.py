@@ -67,5 +67,11 @@ def test_c5_job_matches_twin():
     Pt, Et, good, _ = twin_conditioning(tw, np.concatenate([sp.quat, sp.trans], 1), sp.nsteps, sp.lr_rot,
                                         sp.lr_trans, sp.ligand_coor)
     rr = pose_rmsd(res["pose"], Pt, sp.ligand_coor)
+    de = np.abs(res["energy"] - Et) / np.maximum(np.abs(Et), 1.0)
+    print(f"C5 served job ({len(sp.quat)} poses, {sp.nsteps} steps, {r.coor.shape[0]}-bead receptor) vs the twin: "
+          f"{good.sum()} of {len(good)} poses well conditioned; RMSD max {rr[good].max():.1e} median "
+          f"{np.median(rr[good]):.1e} A; energy rel max {de[good].max():.1e} p99 {np.percentile(de[good], 99):.1e} "
+          f"median {np.median(de[good]):.1e}")
     assert good.mean() >= 0.9
     assert rr[good].max() < 0.05 and np.median(rr) < 0.01
+    assert np.percentile(de[good], 99) < 1e-4
