@@ -1672,10 +1672,50 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int n, 
 // rt within the cutoff (d = point - bead, c = P / |d| for capped beads, else 1).  The bead loop
 // computes the 8 sums once for all types; the pair types whose lower branch can hold get their
 // difference added bead by bead.  Beads are staged in LDS in type order (a stable counting sort
-// per block, so sums run in the same order in every block and every run).
+// per block, so sums run in the same order in every block and every run).  The contraction
+// coefficients (ivor rc, -(ivor ac), -3 ivor ac) of up to kBuildRowTile receptor types are
+// formed once per block into LDS, with a mask per receptor type of the ligand types whose lower
+// branch applies, so a grid point spends 8 FMAs per (receptor type, ligand type).
 constexpr int kBuildTypes = 32;
 constexpr int kBuildChunk = 2048;  // receptor beads staged per pass
 constexpr int kBuildThreads = 256;
+constexpr int kBuildRowTile = 32;  // receptor types whose coefficients are staged at once
+
+// LDS bytes of build_potentials_kernel for nrt receptor types.
+__host__ __device__ __forceinline__ size_t build_lds_bytes(int nrt) {
+  return (size_t)kBuildChunk * 16 + (size_t)kBuildRowTile * kBuildTypes * 16 + (size_t)kBuildRowTile * 4 +
+         (size_t)(2 * nrt + 1 + kBuildThreads) * 4;
+}
+
+// Coefficients of receptor types rt0 .. rt0 + nrow - 1 against the block's ligand types
+// t0 .. t0 + ng - 1: s_coef[r][k] = (ivor rc, -(ivor ac), -3 (ivor ac), 0), the products the
+// contraction multiplies by U4, U3 and V4 (V5's factor 4 ivor rc is taken as ivor rc times 4 V5,
+// the same product), and bit k of s_low[r] set where the lower branch applies (ivor != 1 and
+// rmin2 > P^2).  Ligand types beyond the group get zeros.
+__device__ void stage_build_coefs(const float* rc, const float* ac, const float* ivor, const float* rmin2, int nlt,
+                                  const int* lig_fftype, int t0, int ng, int rt0, int nrow, float P2,
+                                  float4* s_coef, unsigned* s_low) {
+  static_assert(kBuildTypes == 32 && kBuildThreads % kWave == 0, "two coefficient rows per wavefront");
+  const int lane = threadIdx.x % kWave;
+  for (int e0 = threadIdx.x - lane; e0 < kBuildRowTile * kBuildTypes; e0 += blockDim.x) {
+    const int e = e0 + lane, r = e / kBuildTypes, k = e % kBuildTypes;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool low = false;
+    if (r < nrow && k < ng) {
+      // types beyond the pair table are read as its last row / column (memory-safe for any input)
+      const int lt = (int)min((unsigned)lig_fftype[t0 + k], (unsigned)(nlt - 1));
+      const int pi = (rt0 + r) * nlt + lt;
+      const float iv = ivor[pi];
+      const float a = iv * rc[pi], bb = iv * ac[pi];
+      c = make_float4(a, -bb, -3.f * bb, 0.f);
+      low = iv != 1.f && rmin2[pi] > P2;
+    }
+    s_coef[e] = c;
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(low);
+    if (lane == 0) s_low[r] = (unsigned)m;
+    if (lane == kWave / 2) s_low[r] = (unsigned)(m >> 32);
+  }
+}
 
 __device__ void stage_by_type(const float4* rec, int c0, int n, int nrt, float4* s_bead, int* s_off, int* s_cur,
                               int* s_type) {
@@ -1726,7 +1766,9 @@ __global__ void __launch_bounds__(kBuildThreads, 3)
                             int outer) {
   extern __shared__ float4 smem[];
   float4* s_bead = smem;
-  int* s_off = reinterpret_cast<int*>(s_bead + kBuildChunk);
+  float4* s_coef = s_bead + kBuildChunk;
+  unsigned* s_low = reinterpret_cast<unsigned*>(s_coef + kBuildRowTile * kBuildTypes);
+  int* s_off = reinterpret_cast<int*>(s_low + kBuildRowTile);
   int* s_cur = s_off + (nrt + 1);
   int* s_type = s_cur + nrt;
   const int t0 = blockIdx.y * kBuildTypes;
@@ -1747,6 +1789,7 @@ __global__ void __launch_bounds__(kBuildThreads, 3)
   }
   const float P2 = G.p2;
   const float P = sqrtf(P2);
+  const float twoP = 2.f * P;
   float E[kBuildTypes], Fx[kBuildTypes], Fy[kBuildTypes], Fz[kBuildTypes];
 #pragma unroll
   for (int k = 0; k < kBuildTypes; ++k) E[k] = Fx[k] = Fy[k] = Fz[k] = 0.f;
@@ -1754,70 +1797,80 @@ __global__ void __launch_bounds__(kBuildThreads, 3)
     const int n = min(kBuildChunk, nrec - c0);
     __syncthreads();  // the previous chunk's readers are done
     stage_by_type(rec, c0, n, nrt, s_bead, s_off, s_cur, s_type);
-    for (int rt = 0; rt < nrt; ++rt) {
-      const int b0 = s_off[rt], b1 = s_off[rt + 1];
-      if (b0 == b1) continue;
-      float U4 = 0.f, U3 = 0.f, V5[3] = {0.f, 0.f, 0.f}, V4[3] = {0.f, 0.f, 0.f};
-      // unrolled so that several LDS reads of beads are in flight at once
+    for (int rt0 = 0; rt0 < nrt; rt0 += kBuildRowTile) {
+      const int nrow = min(kBuildRowTile, nrt - rt0);
+      __syncthreads();  // the previous tile's readers are done
+      stage_build_coefs(rc, ac, ivor, rmin2, nlt, lig_fftype, t0, ng, rt0, nrow, P2, s_coef, s_low);
+      __syncthreads();
+      for (int rt = rt0; rt < rt0 + nrow; ++rt) {
+        const int b0 = s_off[rt], b1 = s_off[rt + 1];
+        if (b0 == b1) continue;
+        float U4 = 0.f, U3 = 0.f, V5[3] = {0.f, 0.f, 0.f}, V4[3] = {0.f, 0.f, 0.f};
+        // unrolled so that several LDS reads of beads are in flight at once
 #pragma unroll 4
-      for (int b = b0; b < b1; ++b) {
-        const float4 rb = s_bead[b];
-        const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
-        const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-        const bool in = dsq < cutoff2;
-        const bool capped = dsq < P2;
-        const float dd = capped ? P2 : dsq;
-        // hardware reciprocal and reciprocal square root (1 ulp): the grid is data compared
-        // with the FP64 restatement at 1e-4, and IEEE division and square root were most of
-        // this loop's instructions
-        const float u = __builtin_amdgcn_rcpf(dd);
-        const float u3 = (u * u) * u;
-        const float u4 = u3 * u;
-        const float c = capped ? (dsq > 0.f ? P * __builtin_amdgcn_rsqf(dsq) : 0.f) : 1.f;
-        const float w4 = in ? 2.f * c * u4 : 0.f;
-        const float w5 = w4 * u;
-        U4 += in ? u4 : 0.f;
-        U3 += in ? u3 : 0.f;
-        V5[0] = fmaf(w5, ddx, V5[0]);
-        V5[1] = fmaf(w5, ddy, V5[1]);
-        V5[2] = fmaf(w5, ddz, V5[2]);
-        V4[0] = fmaf(w4, ddx, V4[0]);
-        V4[1] = fmaf(w4, ddy, V4[1]);
-        V4[2] = fmaf(w4, ddz, V4[2]);
-      }
+        for (int b = b0; b < b1; ++b) {
+          const float4 rb = s_bead[b];
+          const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+          const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+          const bool capped = dsq < P2;
+          // max(dsq, P^2) within the cutoff; beyond it (and for NaN) +infinity, whose reciprocal
+          // 0 makes u^3, u^4 and both weights of the bead exactly 0.  Hardware reciprocal and
+          // reciprocal square root (1 ulp): the grid is data compared with the FP64 restatement
+          // at 1e-4, and IEEE division and square root were most of this loop's instructions.
+          const float dd = dsq < cutoff2 ? fmaxf(dsq, P2) : INFINITY;
+          const float u = __builtin_amdgcn_rcpf(dd);
+          const float u3 = (u * u) * u;
+          const float u4 = u3 * u;
+          // 2c: c = P / |d| for capped beads (0 at d = 0, where the force has no direction), else 1
+          const float c2 = capped ? (dsq > 0.f ? twoP * __builtin_amdgcn_rsqf(dsq) : 0.f) : 2.f;
+          const float w4 = c2 * u4;
+          const float w5 = w4 * u;
+          U4 += u4;
+          U3 += u3;
+          V5[0] = fmaf(w5, ddx, V5[0]);
+          V5[1] = fmaf(w5, ddy, V5[1]);
+          V5[2] = fmaf(w5, ddz, V5[2]);
+          V4[0] = fmaf(w4, ddx, V4[0]);
+          V4[1] = fmaf(w4, ddy, V4[1]);
+          V4[2] = fmaf(w4, ddz, V4[2]);
+        }
+        // 4 ivor rc V5 as ivor rc (4 V5): scaling by 4 is exact, so the fused products agree
+        const float V5s[3] = {4.f * V5[0], 4.f * V5[1], 4.f * V5[2]};
+        const float4* coef = s_coef + (rt - rt0) * kBuildTypes;
+        const unsigned low = __builtin_amdgcn_readfirstlane(s_low[rt - rt0]);
+        // types k >= ng of the last group have zero coefficients and no low bit; their sums are
+        // not stored
 #pragma unroll
-      for (int k = 0; k < kBuildTypes; ++k) {
-        if (k >= ng) continue;
-        // types beyond the pair table are read as its last row / column (memory-safe for any input)
-        const int lt = (int)min((unsigned)lig_fftype[t0 + k], (unsigned)(nlt - 1));
-        const int pi = rt * nlt + lt;
-        const float iv = ivor[pi];
-        const float a = iv * rc[pi], bb = iv * ac[pi];
-        E[k] = fmaf(a, U4, fmaf(-bb, U3, E[k]));
-        Fx[k] = fmaf(4.f * a, V5[0], fmaf(-3.f * bb, V4[0], Fx[k]));
-        Fy[k] = fmaf(4.f * a, V5[1], fmaf(-3.f * bb, V4[1], Fy[k]));
-        Fz[k] = fmaf(4.f * a, V5[2], fmaf(-3.f * bb, V4[2], Fz[k]));
-        const float r2 = rmin2[pi];
-        if (iv != 1.f && r2 > P2) {
-          // the lower branch below rmin2: energy (1 - ivor)(vlj - emin), force from (1 - ivor) dvlj
-          const float em = emin[pi], crc = rc[pi], cac = ac[pi];
-          for (int b = b0; b < b1; ++b) {
-            const float4 rb = s_bead[b];
-            const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
-            const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-            const bool capped = dsq < P2;
-            const float dd = capped ? P2 : dsq;
-            if (!(dsq < cutoff2) || !(dd < r2)) continue;
-            const float u = __builtin_amdgcn_rcpf(dd);
-            const float u3 = (u * u) * u;
-            const float vlj = (crc * u - cac) * u3;
-            const float dv = -(fmaf(4.f, crc * u, -3.f * cac) * u3) * u;
-            const float c = capped ? (dsq > 0.f ? P * __builtin_amdgcn_rsqf(dsq) : 0.f) : 1.f;
-            const float gs = 2.f * (1.f - iv) * dv * c;
-            E[k] += (1.f - iv) * (vlj - em);
-            Fx[k] = fmaf(-gs, ddx, Fx[k]);
-            Fy[k] = fmaf(-gs, ddy, Fy[k]);
-            Fz[k] = fmaf(-gs, ddz, Fz[k]);
+        for (int k = 0; k < kBuildTypes; ++k) {
+          const float4 cf = coef[k];
+          E[k] = fmaf(cf.x, U4, fmaf(cf.y, U3, E[k]));
+          Fx[k] = fmaf(cf.x, V5s[0], fmaf(cf.z, V4[0], Fx[k]));
+          Fy[k] = fmaf(cf.x, V5s[1], fmaf(cf.z, V4[1], Fy[k]));
+          Fz[k] = fmaf(cf.x, V5s[2], fmaf(cf.z, V4[2], Fz[k]));
+          if (low & (1u << k)) {
+            // the lower branch below rmin2: energy (1 - ivor)(vlj - emin), force from (1 - ivor) dvlj
+            const int lt = (int)min((unsigned)lig_fftype[t0 + k], (unsigned)(nlt - 1));
+            const int pi = rt * nlt + lt;
+            const float iv = ivor[pi], r2 = rmin2[pi];
+            const float em = emin[pi], crc = rc[pi], cac = ac[pi];
+            for (int b = b0; b < b1; ++b) {
+              const float4 rb = s_bead[b];
+              const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
+              const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+              const bool capped = dsq < P2;
+              const float dd = capped ? P2 : dsq;
+              if (!(dsq < cutoff2) || !(dd < r2)) continue;
+              const float u = __builtin_amdgcn_rcpf(dd);
+              const float u3 = (u * u) * u;
+              const float vlj = (crc * u - cac) * u3;
+              const float dv = -(fmaf(4.f, crc * u, -3.f * cac) * u3) * u;
+              const float c = capped ? (dsq > 0.f ? P * __builtin_amdgcn_rsqf(dsq) : 0.f) : 1.f;
+              const float gs = 2.f * (1.f - iv) * dv * c;
+              E[k] += (1.f - iv) * (vlj - em);
+              Fx[k] = fmaf(-gs, ddx, Fx[k]);
+              Fy[k] = fmaf(-gs, ddy, Fy[k]);
+              Fz[k] = fmaf(-gs, ddz, Fz[k]);
+            }
           }
         }
       }
@@ -2403,7 +2456,7 @@ int atj_grid_build_potentials(const AtjGrid* geom, const AtjForceField* ff, cons
   const float c2 = cutoff * cutoff;
   const float4* rb = reinterpret_cast<const float4*>(rec->beads);
   if (rec->nrec == 0) return fail(ATJ_ERR_INVALID, "receptor has no beads");
-  const size_t lds = kBuildChunk * sizeof(float4) + (size_t)(2 * ff->nrt + 1 + kBuildThreads) * sizeof(int);
+  const size_t lds = build_lds_bytes(ff->nrt);
   auto kern = build_potentials_kernel;
   if (lds > 65536)
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
