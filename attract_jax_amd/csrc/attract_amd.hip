@@ -1933,8 +1933,13 @@ __device__ __forceinline__ int shell_group(float d, float r0, float h, int n) {
   return d < r0 ? 0 : min(n + 1, (int)ceilf((d - r0) / h));
 }
 
+// A counting sort in two passes over the list: the first counts the entries of each group,
+// the second writes each entry at its group's running offset.  The n + 2 <= 8 group counters
+// (each below 2^16: a list holds fewer than 65535 entries) are packed four to a 64-bit word,
+// so the group index selects a shift rather than a register.
 __global__ void shell_lists_kernel(GridView G, const float4* rec, int nrec, int n, float r0, float h, int npts,
                                    uint16_t* out, int4* nbshell) {
+  static_assert(kMaxShell + 2 <= 8, "group counters: four 16-bit fields in each of two words");
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= npts) return;
   const int2 pl = G.nbgrid[v];
@@ -1942,21 +1947,38 @@ __global__ void shell_lists_kernel(GridView G, const float4* rec, int nrec, int 
   const float px = fmaf((float)ix, G.spacing, G.ox);
   const float py = fmaf((float)iy, G.spacing, G.oy);
   const float pz = fmaf((float)iz, G.spacing, G.oz);
-  int w = 0;
+  auto group_of = [&](unsigned r) {
+    if (r >= (unsigned)nrec) return n + 1;
+    const float4 rb = rec[r];
+    const float dx = rb.x - px, dy = rb.y - py, dz = rb.z - pz;
+    return shell_group(sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), r0, h, n);
+  };
+  // adds `one` << 16 g to the field of group g
+  auto bump = [](unsigned long long& lo, unsigned long long& hi, int g, unsigned long long one) {
+    const unsigned long long f = one << (16 * (g & 3));
+    lo += g < 4 ? f : 0ull;
+    hi += g < 4 ? 0ull : f;
+  };
+  auto field = [](unsigned long long lo, unsigned long long hi, int g) {
+    return (int)(((g < 4 ? lo : hi) >> (16 * (g & 3))) & 0xffffu);
+  };
+  unsigned long long nlo = 0, nhi = 0;  // entries per group
+  for (int t = 0; t < pl.x; ++t) bump(nlo, nhi, group_of(G.nblist[pl.y + t]), 1ull);
+  // offsets: entries of the groups before; count k of the shells = entries of groups 0..k + 1
+  unsigned long long olo = 0, ohi = 0;
   unsigned c[3] = {0u, 0u, 0u};
+  int w = 0;
   for (int j = 0; j <= n + 1; ++j) {
-    for (int t = 0; t < pl.x; ++t) {
-      const uint16_t r = G.nblist[pl.y + t];
-      int grp = n + 1;
-      if (r < nrec) {
-        const float4 rb = rec[r];
-        const float dx = rb.x - px, dy = rb.y - py, dz = rb.z - pz;
-        grp = shell_group(sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), r0, h, n);
-      }
-      if (grp == j) out[pl.y + w++] = r;
-    }
+    bump(olo, ohi, j, (unsigned long long)w);
+    w += field(nlo, nhi, j);
     const int k = j - 1;  // shell k reads groups 0..k + 1
     if (k >= 0 && k < n) c[k >> 1] |= (unsigned)w << (16 * (k & 1));
+  }
+  for (int t = 0; t < pl.x; ++t) {
+    const uint16_t r = G.nblist[pl.y + t];
+    const int g = group_of(r);
+    out[pl.y + field(olo, ohi, g)] = r;
+    bump(olo, ohi, g, 1ull);
   }
   nbshell[v] = make_int4(pl.y, (int)c[0], (int)c[1], (int)c[2]);
 }
