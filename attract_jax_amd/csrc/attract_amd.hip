@@ -1884,41 +1884,138 @@ __global__ void __launch_bounds__(kBuildThreads, 3)
   }
 }
 
-__global__ void count_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis2, int* counts) {
-  const long long nv = (long long)G.dx * G.dy * G.dz;
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const int iz = (int)(v % G.dz), iy = (int)((v / G.dz) % G.dy), ix = (int)(v / ((long long)G.dy * G.dz));
-  const float px = G.ox + G.spacing * ix, py = G.oy + G.spacing * iy, pz = G.oz + G.spacing * iz;
-  int c = 0;
-  for (int r = 0; r < nrec; ++r) {
-    const float4 rb = rec[r];
-    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
-    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-    c += dsq < nbdis2;
-  }
-  counts[v] = c;
+// Neighbour lists are built over bricks of kBrickX x kBrickY x kBrickZ grid points, one point
+// per thread of a block.  The block stages, in receptor order, the beads that can lie within
+// neighbourdis of some point of its brick (within neighbourdis + the brick's half diagonal +
+// 0.01 A of the brick's centre, a margin far above the rounding of these float distances), and
+// each point tests only those with the same arithmetic as a test against every bead: the
+// counts and lists are those of the whole receptor.
+constexpr int kBrickX = 4, kBrickY = 8, kBrickZ = 8;
+constexpr int kBrickThreads = kBrickX * kBrickY * kBrickZ;
+constexpr int kListCand = 2048;  // candidate beads staged per pass
+
+struct ListBrick {
+  long long v;  // the thread's grid point (flat index), -1 beyond the grid
+  float px, py, pz;
+  float cx, cy, cz, cull2;
+};
+
+__host__ __device__ __forceinline__ long long brick_count(int dx, int dy, int dz) {
+  return (long long)((dx + kBrickX - 1) / kBrickX) * ((dy + kBrickY - 1) / kBrickY) * ((dz + kBrickZ - 1) / kBrickZ);
 }
 
-__global__ void fill_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis2, const int* counts,
-                                       const long long* starts, int2* nbgrid, uint16_t* nblist) {
-  const long long nv = (long long)G.dx * G.dy * G.dz;
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const int iz = (int)(v % G.dz), iy = (int)((v / G.dz) % G.dy), ix = (int)(v / ((long long)G.dy * G.dz));
-  const float px = G.ox + G.spacing * ix, py = G.oy + G.spacing * iy, pz = G.oz + G.spacing * iz;
-  const long long s = starts[v];
-  const int n = counts[v];
-  int c = 0;
-  for (int r = 0; r < nrec && c < n; ++r) {
-    const float4 rb = rec[r];
-    const float ddx = px - rb.x, ddy = py - rb.y, ddz = pz - rb.z;
-    const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-    if (dsq < nbdis2) nblist[s + c++] = (uint16_t)r;
+__device__ ListBrick list_brick(const GridView& G, float nbdis) {
+  const int nby = (G.dy + kBrickY - 1) / kBrickY, nbz = (G.dz + kBrickZ - 1) / kBrickZ;
+  const long long b = blockIdx.x;
+  const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), bx = (int)(b / ((long long)nbz * nby));
+  const int t = threadIdx.x;
+  const int ix = bx * kBrickX + t / (kBrickY * kBrickZ), iy = by * kBrickY + (t / kBrickZ) % kBrickY,
+            iz = bz * kBrickZ + t % kBrickZ;
+  ListBrick k;
+  // grid point coordinates exactly as the per-point test has them
+  k.px = G.ox + G.spacing * ix;
+  k.py = G.oy + G.spacing * iy;
+  k.pz = G.oz + G.spacing * iz;
+  k.v = (ix < G.dx && iy < G.dy && iz < G.dz) ? ((long long)ix * G.dy + iy) * G.dz + iz : -1;
+  const int x1 = min(bx * kBrickX + kBrickX, G.dx) - 1, y1 = min(by * kBrickY + kBrickY, G.dy) - 1,
+            z1 = min(bz * kBrickZ + kBrickZ, G.dz) - 1;
+  const float lx = G.ox + G.spacing * (bx * kBrickX), hx = G.ox + G.spacing * x1;
+  const float ly = G.oy + G.spacing * (by * kBrickY), hy = G.oy + G.spacing * y1;
+  const float lz = G.oz + G.spacing * (bz * kBrickZ), hz = G.oz + G.spacing * z1;
+  k.cx = 0.5f * (lx + hx);
+  k.cy = 0.5f * (ly + hy);
+  k.cz = 0.5f * (lz + hz);
+  const float ex = 0.5f * (hx - lx), ey = 0.5f * (hy - ly), ez = 0.5f * (hz - lz);
+  // |nbdis|: the point test depends on its square only.  The margin covers the rounding of
+  // both distances, relative to the coordinates' magnitude, with 0.01 A to spare.
+  const float reach = fabsf(nbdis) + sqrtf(fmaf(ez, ez, fmaf(ey, ey, ex * ex)));
+  const float mag = fabsf(k.cx) + fabsf(k.cy) + fabsf(k.cz) + reach;
+  const float cull = fmaf(mag, 1e-5f, reach + 0.01f);
+  k.cull2 = cull * cull;
+  return k;
+}
+
+// Stages into s_cand, in receptor order, the beads from *pos on that lie within the brick's
+// cull radius (x, y, z and the receptor index as int bits), until the next round of
+// blockDim.x beads might not fit or the receptor ends; *pos moves past the beads examined.
+// Returns the number staged.  Every thread of the block calls it; it ends with a barrier.
+__device__ int stage_candidates(const float4* rec, int nrec, int& pos, const ListBrick& k, float4* s_cand,
+                                int* s_wcount) {
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave, nw = blockDim.x / kWave;
+  int n = 0;
+  while (pos < nrec && n + (int)blockDim.x <= kListCand) {
+    const int r = pos + tid;
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool keep = false;
+    if (r < nrec) {
+      b = rec[r];
+      const float dx = b.x - k.cx, dy = b.y - k.cy, dz = b.z - k.cz;
+      keep = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < k.cull2;
+    }
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+    if (lane == 0) s_wcount[wave] = __builtin_popcountll(m);
+    __syncthreads();
+    int before = n, total = n;
+    for (int w = 0; w < nw; ++w) {
+      const int cw = s_wcount[w];
+      before += w < wave ? cw : 0;
+      total += cw;
+    }
+    if (keep) {
+      const unsigned long long below = lane ? m & (~0ull >> (kWave - lane)) : 0ull;
+      s_cand[before + __builtin_popcountll(below)] = make_float4(b.x, b.y, b.z, __int_as_float(r));
+    }
+    __syncthreads();
+    n = total;
+    pos += blockDim.x;
   }
+  return n;
+}
+
+__global__ void __launch_bounds__(kBrickThreads)
+    count_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis, float nbdis2, int* counts) {
+  __shared__ float4 s_cand[kListCand];
+  __shared__ int s_wcount[kBrickThreads / kWave];
+  const ListBrick k = list_brick(G, nbdis);
+  int c = 0;
+  for (int pos = 0; pos < nrec;) {
+    __syncthreads();  // the previous pass's readers are done
+    const int n = stage_candidates(rec, nrec, pos, k, s_cand, s_wcount);
+    for (int i = 0; i < n; ++i) {
+      const float4 rb = s_cand[i];
+      const float ddx = k.px - rb.x, ddy = k.py - rb.y, ddz = k.pz - rb.z;
+      const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+      c += dsq < nbdis2;
+    }
+  }
+  if (k.v >= 0) counts[k.v] = c;
+}
+
+// The first counts[v] beads within neighbourdis of point v, in receptor order, then 0xFFFF
+// padding to a multiple of kListPad entries.
+__global__ void __launch_bounds__(kBrickThreads)
+    fill_neighbours_kernel(GridView G, const float4* rec, int nrec, float nbdis, float nbdis2, const int* counts,
+                           const long long* starts, int2* nbgrid, uint16_t* nblist) {
+  __shared__ float4 s_cand[kListCand];
+  __shared__ int s_wcount[kBrickThreads / kWave];
+  const ListBrick k = list_brick(G, nbdis);
+  const long long s = k.v >= 0 ? starts[k.v] : 0;
+  const int n = k.v >= 0 ? counts[k.v] : 0;
+  int c = 0;
+  for (int pos = 0; pos < nrec;) {
+    __syncthreads();  // the previous pass's readers are done
+    const int nc = stage_candidates(rec, nrec, pos, k, s_cand, s_wcount);
+    for (int i = 0; i < nc; ++i) {
+      const float4 rb = s_cand[i];
+      const float ddx = k.px - rb.x, ddy = k.py - rb.y, ddz = k.pz - rb.z;
+      const float dsq = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+      if (dsq < nbdis2 && c < n) nblist[s + c++] = (uint16_t)__float_as_int(rb.w);
+    }
+  }
+  if (k.v < 0) return;
   const int padded = (n + kListPad - 1) / kListPad * kListPad;
-  for (int k = n; k < padded; ++k) nblist[s + k] = 0xFFFFu;
-  nbgrid[v] = make_int2(n, (int)s);
+  for (int j = n; j < padded; ++j) nblist[s + j] = 0xFFFFu;
+  nbgrid[k.v] = make_int2(n, (int)s);
 }
 
 // Shell order of one grid point's list (atj_grid_shell_lists).  Entry r at distance d
@@ -2525,9 +2622,10 @@ int atj_grid_count_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float
   int rc = make_view(geom, G, false);
   if (rc) return rc;
   if (!rec || !counts) return fail(ATJ_ERR_INVALID, "NULL argument");
-  const long long nv = (long long)G.dx * G.dy * G.dz;
-  hipLaunchKernelGGL(count_neighbours_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                     G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis * neighbourdis, counts);
+  if (rec->nrec < 0 || (rec->nrec > 0 && !rec->beads)) return fail(ATJ_ERR_INVALID, "receptor beads NULL");
+  hipLaunchKernelGGL(count_neighbours_kernel, dim3((unsigned)brick_count(G.dx, G.dy, G.dz)), dim3(kBrickThreads), 0,
+                     (hipStream_t)stream, G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis,
+                     neighbourdis * neighbourdis, counts);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }
@@ -2538,10 +2636,11 @@ int atj_grid_fill_neighbours(const AtjGrid* geom, const AtjReceptor* rec, float 
   int rc = make_view(geom, G, false);
   if (rc) return rc;
   if (!rec || !counts || !starts || !nbgrid || !nblist) return fail(ATJ_ERR_INVALID, "NULL argument");
-  const long long nv = (long long)G.dx * G.dy * G.dz;
-  hipLaunchKernelGGL(fill_neighbours_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                     G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis * neighbourdis, counts,
-                     reinterpret_cast<const long long*>(starts), reinterpret_cast<int2*>(nbgrid), nblist);
+  if (rec->nrec < 0 || (rec->nrec > 0 && !rec->beads)) return fail(ATJ_ERR_INVALID, "receptor beads NULL");
+  hipLaunchKernelGGL(fill_neighbours_kernel, dim3((unsigned)brick_count(G.dx, G.dy, G.dz)), dim3(kBrickThreads), 0,
+                     (hipStream_t)stream, G, reinterpret_cast<const float4*>(rec->beads), rec->nrec, neighbourdis,
+                     neighbourdis * neighbourdis, counts, reinterpret_cast<const long long*>(starts),
+                     reinterpret_cast<int2*>(nbgrid), nblist);
   HIPCHK(hipGetLastError());
   return ATJ_OK;
 }
