@@ -51,7 +51,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kWave = 64;
 constexpr int kListPad = 16;       // lists are padded with 0xFFFF to a multiple of this
-constexpr int kHalf = 32;          // ligands of at most this many beads take half a wave in the bead phase
+constexpr int kHalf = 32;          // poses of at most this many beads share a bead phase two by two
 constexpr int kMaxShell = 6;       // most list shells per grid point (16-bit counts in one int4)
 constexpr double kShellMargin = 1e-3;  // A added to every shell radius: covers FP32 rounding of distances
 constexpr int kMaxWavesPerBlock = 16;
@@ -548,6 +548,12 @@ struct WaveScratch {
   float4 gridB[kHalf];       // grid energy and gradient of the beads of the second pose of a shared bead phase
 };
 
+// Whether two consecutive poses of na and nb beads share one bead phase: both of at most kHalf
+// beads.  pose_energrad takes any pair whose smaller pose has at most kHalf beads and whose
+// beads fit the wave, but pairing large poses with small ones, although it saved 6% of the
+// minimizer's VALU instructions, made its launches 3% slower (more time waiting on results).
+__device__ __forceinline__ bool share_bead_phase(int na, int nb) { return na <= kHalf && nb <= kHalf; }
+
 // Minimizer batch of one wavefront, in LDS: the pose slots whose energies the wave
 // evaluates in order (two small ones at a time where they share a bead phase), and whose
 // optimizer updates it then runs together (lane 8 slot + parameter).
@@ -836,12 +842,12 @@ struct PoseIn {
 // Energy and dE/dM of pose A, or of poses A and B.  Returns the wave-reduced sums in the
 // transposed layout of wave_sum_all: lane l holds accumulator acc_index(l) (0: energy,
 // 1 + 3k + l: dE/dM[k][l], 10 + l: dE/dM[3][l]) of A in uA and of B in uB.
-// A pose of more than kHalf beads (B.nb must be 0) takes the whole wave, in chunks of 64
-// beads.  A pose of at most kHalf beads puts bead j on lane j; with a second such pose B its
-// beads run on lanes 32 + j in the same bead phase, and its grid sums wait in LDS while A's
-// pairs are evaluated, then move to lanes 0-31.  Either way a pose's sums are formed on the
-// same lanes in the same order (grid terms of bead j on lane j, then its pairs in windows of
-// its own, one reduction), so its result is the same alone or beside another.  With energy
+// A pose alone puts bead c0 + j on lane j, in chunks of 64 beads.  With a second pose B
+// (B.nb <= kHalf and A.nb + B.nb <= 64; the callers pair by share_bead_phase) both run in one bead
+// phase, A's bead j on lane j and B's on lane 64 - B.nb + j; B's grid sums wait in LDS while
+// A's pairs are evaluated, then move to lanes 0 .. B.nb - 1.  Either way a pose's sums are
+// formed on the same lanes in the same order (grid terms of bead j on lane j, then its pairs
+// in windows of its own, one reduction), so its result is the same alone or beside another.  With energy
 // false (minimizer steps whose energy is not kept) the pair energies are left out of the
 // energy sum; every gradient sum is the same.
 template <int MODE, bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
@@ -853,9 +859,9 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
   const GridView& G = a.G;
-  const bool halves = A.nb <= kHalf;
-  const int nposes = halves && B.nb > 0 ? 2 : 1;
-  const int h = halves ? lane >> 5 : 0;  // pose of this lane in the bead phase
+  const int nposes = B.nb > 0 ? 2 : 1;
+  const int sp = nposes == 2 ? kWave - B.nb : kWave;  // B's first lane
+  const int h = lane >= sp ? 1 : 0;                    // pose of this lane in the bead phase
   float acc[kAcc];
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
@@ -865,7 +871,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   // a pose without beads has energy and gradient 0; paired with B it still runs B's chunk
   const int nloop = max(A.nb, nposes == 2 ? B.nb : 0);
   for (int c0 = 0; c0 < nloop; c0 += kWave) {
-    const int j = halves ? lane & (kHalf - 1) : c0 + lane;
+    const int j = h ? lane - sp : c0 + lane;
     const int nbh = h ? B.nb : A.nb;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
@@ -905,7 +911,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     }
     phase_end(st, kPhBead);
     if (nposes == 2) {  // B's bead terms wait in LDS; its lanes start A's pair sums from zero
-      if (h) ws->gridB[lane - kHalf] = make_float4(Eg, g[0], g[1], g[2]);
+      if (h) ws->gridB[lane - sp] = make_float4(Eg, g[0], g[1], g[2]);
 #pragma unroll
       for (int i = 0; i < kAcc; ++i) acc[i] = h ? 0.f : acc[i];
     }
@@ -916,11 +922,10 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const int seg8 = (cnt + 7) >> 3, seg4 = (cnt + 3) >> 2;
       const int incl8 = wave_incl_scan(seg8);
       const int incl4 = wave_incl_scan(seg4);
-      const int endA = halves ? kHalf - 1 : kWave - 1;
-      t8A = __builtin_amdgcn_readlane(incl8, endA);
-      t4A = __builtin_amdgcn_readlane(incl4, endA);
-      t8B = halves ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
-      t4B = halves ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
+      t8A = __builtin_amdgcn_readlane(incl8, sp - 1);
+      t4A = __builtin_amdgcn_readlane(incl4, sp - 1);
+      t8B = nposes == 2 ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
+      t4B = nposes == 2 ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
       u8A = seg8_cost(t8A) <= ((t4A + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
       u8B = seg8_cost(t8B) <= ((t4B + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
       const bool use8 = h ? u8B : u8A;
@@ -937,13 +942,13 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
     phase_end(st, kPhScan);
     unsigned npair = 0;  // wave-wide count, credited to lane 0
     for (int q = 0; q < nposes; ++q) {
-      if (q == 1 && !is_adjoint(MODE)) {  // A is complete; B's bead terms to lanes 0-31, summed as in B's own bead phase
+      if (q == 1 && !is_adjoint(MODE)) {  // A is complete; B's bead terms to lanes 0 .. B.nb - 1, summed as in B's own bead phase
         uA = wave_sum_all(acc, lane);
 #pragma unroll
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
-        if (!h) {
+        if (lane < B.nb) {
           const float4 e = ws->gridB[lane];
-          const float4 x = ws->xl[kHalf + lane];  // -8 x_local
+          const float4 x = ws->xl[sp + lane];  // -8 x_local
           const float gb[3] = {e.y, e.z, e.w}, xb[3] = {-0.125f * x.x, -0.125f * x.y, -0.125f * x.z};
           bead_acc(acc, e.x, gb, xb);
         }
@@ -1237,9 +1242,9 @@ __device__ __forceinline__ Claim first_claim(const KArgs& a, int lane) {
 
 // Minimizer loop of one wavefront.  The wave takes kBatch consecutive poses of the
 // processing order at a time; per optimizer step it evaluates the slots still stepping
-// in order (all 64 lanes on one pose, or a slot of at most kHalf beads together with the
-// next such slot), then lane 8 b + i updates parameter i of slot b.  Slots may belong to
-// different jobs and have different step counts.
+// in order (a slot alone, or together with the next stepping slot when the two share a bead
+// phase), then lane 8 b + i updates parameter i of slot b.  Slots may belong to different
+// jobs and have different step counts.
 template <bool REC_LDS, bool PLATEAU, bool EXACT, int LAYOUT>
 __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, const float4* __restrict__ s_par,
                               const float2* __restrict__ s_side, const float* __restrict__ s_kh, WaveScratch* ws,
@@ -1289,26 +1294,32 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
     wave_sync();
     phase_end(st, kPhSetup);
     for (int s = 0; s < nst_max; ++s) {
-      // active slots in order; two consecutive ones of at most kHalf beads share the bead phase
+      // active slots in order; a slot and the next stepping one share the bead phase when they
+      // fit it (the smaller as pose B)
       for (int b = 0; b < nbatch; ++b) {
         if (s >= __builtin_amdgcn_readlane(nst, b)) continue;
+        const int nbb = uni(bs->nb[b]);
+        int c = b + 1;
+        while (c < nbatch && s >= __builtin_amdgcn_readlane(nst, c)) ++c;
+        const int partner = c < nbatch && share_bead_phase(nbb, uni(bs->nb[c])) ? c : -1;
+        // slots of poses A and B
+        int sa = b, sbb = -1;
+        if (partner >= 0) {
+          const bool swap = uni(bs->nb[partner]) > nbb;
+          sa = swap ? partner : b;
+          sbb = swap ? b : partner;
+        }
         PoseIn P[2];
-        int partner = -1;
-        P[0].lig = bs->lig[b];
-        P[0].nb = uni(bs->nb[b]);
-        P[0].M = bs->M[b];
+        P[0].lig = bs->lig[sa];
+        P[0].nb = uni(bs->nb[sa]);
+        P[0].M = bs->M[sa];
         P[0].wgt = 0.f;
         P[1] = P[0];
         P[1].nb = 0;
-        if (P[0].nb <= kHalf) {
-          int c = b + 1;
-          while (c < nbatch && s >= __builtin_amdgcn_readlane(nst, c)) ++c;
-          if (c < nbatch && uni(bs->nb[c]) <= kHalf) {
-            partner = c;
-            P[1].lig = bs->lig[c];
-            P[1].nb = uni(bs->nb[c]);
-            P[1].M = bs->M[c];
-          }
+        if (sbb >= 0) {
+          P[1].lig = bs->lig[sbb];
+          P[1].nb = uni(bs->nb[sbb]);
+          P[1].M = bs->M[sbb];
         }
         // the energy is kept only from a slot's last step of the launch
         bool energy = s + 1 >= __builtin_amdgcn_readlane(nst, b);
@@ -1318,8 +1329,8 @@ __device__ void minimize_wave(const KArgs& a, const float4* __restrict__ s_rec, 
                                                                   P[1], st, lane, uA, uB, energy);
         const int i = acc_index(lane);
         if (!(lane & 3) && i < kAcc) {
-          bs->acc[b][i] = uA;
-          if (partner >= 0) bs->acc[partner][i] = uB;
+          bs->acc[sa][i] = uA;
+          if (sbb >= 0) bs->acc[sbb][i] = uB;
         }
         if (partner >= 0) b = partner;
       }
@@ -1406,8 +1417,8 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock)
     BatchScratch* bs = reinterpret_cast<BatchScratch*>(s_ws + nwaves) + wave;
     minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, ws, bs, st, lane);
   } else {
-    // a.batch poses of the processing order per work-counter claim; two consecutive ones of at
-    // most kHalf beads share the bead phase.  Lane b fetches pose b's index, weight and matrix
+    // a.batch poses of the processing order per claim; two consecutive ones share the bead phase
+    // when they fit it (the smaller as pose B).  Lane b fetches pose b's index, weight and matrix
     // for the whole claim at once (one round of memory latency per claim, not per pose).
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
     float* Ms = s_mat + wave * (kBatch * 12);
@@ -1433,27 +1444,35 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock)
       }
       wave_sync();
       for (int b = 0; b < nbatch; ++b) {
+        // this pose and, when the two share the bead phase, the next: the larger as pose A
+        const int pb = __builtin_amdgcn_readlane(p_l, b);
+        const int nbb = a.jobs[a.pose_job ? a.pose_job[pb] : 0].ligand.nbeads;
+        int nbc = 0;
+        if (b + 1 < nbatch) {
+          const int pc = __builtin_amdgcn_readlane(p_l, b + 1);
+          nbc = a.jobs[a.pose_job ? a.pose_job[pc] : 0].ligand.nbeads;
+        }
+        const bool pair = b + 1 < nbatch && share_bead_phase(nbb, nbc);
+        const int sa = pair && nbc > nbb ? b + 1 : b;  // claim slots of poses A and B
+        const int sbb = pair ? (sa == b ? b + 1 : b) : -1;
         int p[2];
         PoseIn P[2];
-        p[0] = __builtin_amdgcn_readlane(p_l, b);
+        p[0] = __builtin_amdgcn_readlane(p_l, sa);
         const AtjJob job = a.jobs[a.pose_job ? a.pose_job[p[0]] : 0];
         P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
         P[0].nb = job.ligand.nbeads;
-        P[0].M = Ms + 12 * b;
-        P[0].wgt = is_adjoint(MODE) ? lane_value(w_l, b) : 0.f;
+        P[0].M = Ms + 12 * sa;
+        P[0].wgt = is_adjoint(MODE) ? lane_value(w_l, sa) : 0.f;
         P[1] = P[0];
         P[1].nb = 0;
         p[1] = -1;
-        if (P[0].nb <= kHalf && b + 1 < nbatch) {
-          const int q = __builtin_amdgcn_readlane(p_l, b + 1);
-          const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[q] : 0];
-          if (job2.ligand.nbeads <= kHalf) {
-            p[1] = q;
-            P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
-            P[1].nb = job2.ligand.nbeads;
-            P[1].M = Ms + 12 * (b + 1);
-            P[1].wgt = is_adjoint(MODE) ? lane_value(w_l, b + 1) : 0.f;
-          }
+        if (pair) {
+          p[1] = __builtin_amdgcn_readlane(p_l, sbb);
+          const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[p[1]] : 0];
+          P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
+          P[1].nb = job2.ligand.nbeads;
+          P[1].M = Ms + 12 * sbb;
+          P[1].wgt = is_adjoint(MODE) ? lane_value(w_l, sbb) : 0.f;
         }
         float u[2];
         pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, w_pgrad, ws, P[0], P[1], st,
