@@ -859,9 +859,14 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   const bool do_grid = (a.flags & ATJ_TERM_GRID) != 0;
   const bool do_nb = (a.flags & ATJ_TERM_NB) != 0;
   const GridView& G = a.G;
-  const int nposes = B.nb > 0 ? 2 : 1;
-  const int sp = nposes == 2 ? kWave - B.nb : kWave;  // B's first lane
-  const int h = lane >= sp ? 1 : 0;                    // pose of this lane in the bead phase
+  // B's first lane sp: the minimizer puts B on the last B.nb lanes, the single-shot modes on
+  // lanes 32 + j (same sums either way; each form measured faster in its mode, whose kernels
+  // use every one of their 128 registers)
+  const bool halves = A.nb <= kHalf;
+  const int nposes = B.nb > 0 ? 2 : 1;  // the callers pair poses of at most kHalf beads only
+  const bool min_form = MODE == kMinimize;
+  const int sp = min_form ? (nposes == 2 ? kWave - B.nb : kWave) : (halves ? kHalf : kWave);
+  const int h = min_form ? (lane >= sp ? 1 : 0) : (halves ? lane >> 5 : 0);  // pose of this lane
   float acc[kAcc];
 #pragma unroll
   for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
@@ -871,7 +876,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
   // a pose without beads has energy and gradient 0; paired with B it still runs B's chunk
   const int nloop = max(A.nb, nposes == 2 ? B.nb : 0);
   for (int c0 = 0; c0 < nloop; c0 += kWave) {
-    const int j = h ? lane - sp : c0 + lane;
+    const int j = min_form ? (h ? lane - sp : c0 + lane) : (halves ? lane & (kHalf - 1) : c0 + lane);
     const int nbh = h ? B.nb : A.nb;
     int cnt = 0, start = 0;
     float xl[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
@@ -924,8 +929,9 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
       const int incl4 = wave_incl_scan(seg4);
       t8A = __builtin_amdgcn_readlane(incl8, sp - 1);
       t4A = __builtin_amdgcn_readlane(incl4, sp - 1);
-      t8B = nposes == 2 ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
-      t4B = nposes == 2 ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
+      const bool two = min_form ? nposes == 2 : halves;
+      t8B = two ? __builtin_amdgcn_readlane(incl8, kWave - 1) - t8A : 0;
+      t4B = two ? __builtin_amdgcn_readlane(incl4, kWave - 1) - t4A : 0;
       u8A = seg8_cost(t8A) <= ((t4A + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
       u8B = seg8_cost(t8B) <= ((t4B + kWave - 1) / kWave) * (4 * kPairCost + kItemCost);
       const bool use8 = h ? u8B : u8A;
@@ -946,7 +952,7 @@ __device__ void pose_energrad(const KArgs& a, const float4* __restrict__ s_rec, 
         uA = wave_sum_all(acc, lane);
 #pragma unroll
         for (int i = 0; i < kAcc; ++i) acc[i] = 0.f;
-        if (lane < B.nb) {
+        if (min_form ? lane < B.nb : !h) {
           const float4 e = ws->gridB[lane];
           const float4 x = ws->xl[sp + lane];  // -8 x_local
           const float gb[3] = {e.y, e.z, e.w}, xb[3] = {-0.125f * x.x, -0.125f * x.y, -0.125f * x.z};
@@ -1418,7 +1424,7 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock)
     minimize_wave<REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, ws, bs, st, lane);
   } else {
     // a.batch poses of the processing order per claim; two consecutive ones share the bead phase
-    // when they fit it (the smaller as pose B).  Lane b fetches pose b's index, weight and matrix
+    // when share_bead_phase allows.  Lane b fetches pose b's index, weight and matrix
     // for the whole claim at once (one round of memory latency per claim, not per pose).
     const bool qt = (a.flags & ATJ_POSE_QT) != 0;
     float* Ms = s_mat + wave * (kBatch * 12);
@@ -1444,35 +1450,28 @@ __global__ void __launch_bounds__(kWave * kMaxWavesPerBlock)
       }
       wave_sync();
       for (int b = 0; b < nbatch; ++b) {
-        // this pose and, when the two share the bead phase, the next: the larger as pose A
-        const int pb = __builtin_amdgcn_readlane(p_l, b);
-        const int nbb = a.jobs[a.pose_job ? a.pose_job[pb] : 0].ligand.nbeads;
-        int nbc = 0;
-        if (b + 1 < nbatch) {
-          const int pc = __builtin_amdgcn_readlane(p_l, b + 1);
-          nbc = a.jobs[a.pose_job ? a.pose_job[pc] : 0].ligand.nbeads;
-        }
-        const bool pair = b + 1 < nbatch && share_bead_phase(nbb, nbc);
-        const int sa = pair && nbc > nbb ? b + 1 : b;  // claim slots of poses A and B
-        const int sbb = pair ? (sa == b ? b + 1 : b) : -1;
         int p[2];
         PoseIn P[2];
-        p[0] = __builtin_amdgcn_readlane(p_l, sa);
+        p[0] = __builtin_amdgcn_readlane(p_l, b);
         const AtjJob job = a.jobs[a.pose_job ? a.pose_job[p[0]] : 0];
         P[0].lig = reinterpret_cast<const float4*>(job.ligand.beads);
         P[0].nb = job.ligand.nbeads;
-        P[0].M = Ms + 12 * sa;
-        P[0].wgt = is_adjoint(MODE) ? lane_value(w_l, sa) : 0.f;
+        P[0].M = Ms + 12 * b;
+        P[0].wgt = is_adjoint(MODE) ? lane_value(w_l, b) : 0.f;
         P[1] = P[0];
         P[1].nb = 0;
         p[1] = -1;
-        if (pair) {
-          p[1] = __builtin_amdgcn_readlane(p_l, sbb);
-          const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[p[1]] : 0];
-          P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
-          P[1].nb = job2.ligand.nbeads;
-          P[1].M = Ms + 12 * sbb;
-          P[1].wgt = is_adjoint(MODE) ? lane_value(w_l, sbb) : 0.f;
+        // the next pose is read only when this one can share its bead phase
+        if (P[0].nb <= kHalf && b + 1 < nbatch) {
+          const int q = __builtin_amdgcn_readlane(p_l, b + 1);
+          const AtjJob job2 = a.jobs[a.pose_job ? a.pose_job[q] : 0];
+          if (share_bead_phase(P[0].nb, job2.ligand.nbeads)) {
+            p[1] = q;
+            P[1].lig = reinterpret_cast<const float4*>(job2.ligand.beads);
+            P[1].nb = job2.ligand.nbeads;
+            P[1].M = Ms + 12 * (b + 1);
+            P[1].wgt = is_adjoint(MODE) ? lane_value(w_l, b + 1) : 0.f;
+          }
         }
         float u[2];
         pose_energrad<MODE, REC_LDS, PLATEAU, EXACT, LAYOUT>(a, s_rec, s_par, s_side, s_kh, w_pgrad, ws, P[0], P[1], st,
