@@ -844,8 +844,9 @@ struct PoseIn {
 // 1 + 3k + l: dE/dM[k][l], 10 + l: dE/dM[3][l]) of A in uA and of B in uB.
 // A pose alone puts bead c0 + j on lane j, in chunks of 64 beads.  With a second pose B
 // (B.nb <= kHalf and A.nb + B.nb <= 64; the callers pair by share_bead_phase) both run in one bead
-// phase, A's bead j on lane j and B's on lane 64 - B.nb + j; B's grid sums wait in LDS while
-// A's pairs are evaluated, then move to lanes 0 .. B.nb - 1.  Either way a pose's sums are
+// phase, A's bead j on lane j and B's on lane sp + j (sp = 64 - B.nb in the minimizer, 32 in the
+// single-shot modes); B's grid sums wait in LDS while A's pairs are evaluated, then move to lanes
+// 0 .. B.nb - 1.  Either way a pose's sums are
 // formed on the same lanes in the same order (grid terms of bead j on lane j, then its pairs
 // in windows of its own, one reduction), so its result is the same alone or beside another.  With energy
 // false (minimizer steps whose energy is not kept) the pair energies are left out of the
