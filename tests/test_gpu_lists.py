@@ -70,3 +70,45 @@ def test_brick_lists_match_the_oracle():
             assert counts.max() > 2048  # the staging pass limit of a brick
         else:
             assert (counts == 0).mean() > 0.3  # bricks far from the receptor
+
+
+@pytest.mark.gpu
+def test_shell_order_keeps_every_pair_within_the_plateau():
+    """atj_grid_shell_lists reorders each list into distance groups and records, per shell of bead
+    positions in the point's cell, how many leading entries a bead there reads.  For random bead
+    positions, every receptor bead within plateaudis (FP64) must be among those entries, and each
+    reordered list must hold the same entries as before."""
+    from attract_jax_amd.workloads import c5_receptor
+    r = c5_receptor(("shells", 2000), 2000, np.random.default_rng(2000))
+    rng = np.random.default_rng(12)
+    rc, ac, ivor, _, _ = random_pair_table(rng)
+    ff = ForceField.create(rc, ac, ivor, device=DEV)
+    rec = Receptor.create(r.coor, r.types - 1, DEV)
+    g = DeviceGrid.build(rec, ff, np.arange(32), device=DEV, potentials=False, layout=_lib.ATJ_LAYOUT_PLAIN,
+                         **r.grid_kw)
+    whole = g.nblist.cpu().numpy().view(np.uint16).copy()
+    nbgrid = g.nbgrid.cpu().numpy()
+    nshell = 6
+    g.shell_lists(rec, nshell)
+    shelled = g.nblist.cpu().numpy().view(np.uint16)
+    nbshell = g.nbshell.cpu().numpy().view(np.uint16).reshape(-1, 8)  # start (2 words), c0..c5
+    P, h = g.plateaudis, g.spacing * np.sqrt(3.0) / (2 * nshell)
+    coor = np.asarray(r.coor, np.float64)
+    V = int(np.prod(g.dim))
+    counts = nbgrid[:, 0]
+    sample = rng.choice(np.nonzero(counts > 0)[0], 300, replace=False)
+    checked = 0
+    for v in sample:
+        n, s = int(nbgrid[v, 0]), int(nbgrid[v, 1])
+        assert np.array_equal(np.sort(whole[s:s + n]), np.sort(shelled[s:s + n]))
+        p = np.asarray(g.origin, np.float64) + g.spacing * np.array(np.unravel_index(v, g.dim), np.float64)
+        for _ in range(8):
+            x = p + g.spacing * rng.uniform(-0.5, 0.5, 3)
+            k = min(int(np.linalg.norm(x - p) / h), nshell - 1)
+            read = set(shelled[s:s + int(nbshell[v, 2 + k])].tolist())
+            near = np.nonzero(np.linalg.norm(coor - x, axis=1) < P)[0]
+            # neighbourdis covers plateaudis plus the cell's half diagonal: all of them are listed
+            assert set(near.tolist()) <= set(whole[s:s + n].tolist()), v
+            assert set(near.tolist()) <= read, (v, k)
+            checked += len(near)
+    assert checked > 1000
