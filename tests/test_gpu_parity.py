@@ -574,3 +574,35 @@ def test_pose_without_beads_next_to_a_small_one(s1):
     assert torch.equal(st.pose[half:], sa.pose) and torch.equal(st.energy[half:], sa.energy)
     assert torch.count_nonzero(st.energy[:half]) == 0
     assert torch.equal(st.pose[:half, 4:], torch.as_tensor(t[:half], dtype=torch.float32, device=DEV))
+
+
+def _fp64_voxels(mat, coor, grid):
+    """Inner voxel of every bead in FP64 (test-grid-energrad.py:216-222,246-253), -1 outside, and
+    the distance in voxel units to the nearest cell boundary or inner-grid limit."""
+    v, inside = O.voxelize(O.transform(mat.astype(np.float64), coor), grid.origin, grid.gridspacing, grid.dim)
+    pos = np.floor(v + 0.5).astype(np.int64)
+    dy, dz = grid.dim[1], grid.dim[2]
+    flat = np.where(inside, (pos[..., 0] * dy + pos[..., 1]) * dz + pos[..., 2], -1)
+    w = v + 0.5
+    edge = np.minimum(np.abs(w - np.round(w)), np.minimum(np.abs(v), np.abs(v - (np.asarray(grid.dim) - 1))))
+    return flat, edge.min(axis=-1)
+
+
+@pytest.mark.parametrize("which", ["s1", "s2"])
+def test_voxel_boundary_flips_vs_fp64(which, request):
+    """The FP32 path's voxels (bit-exact with the twin) against FP64 arithmetic on the same
+    matrices: the count of beads whose voxel differs, each of them at a cell boundary."""
+    s = request.getfixturevalue(which)
+    mat = s.mats()
+    if which == "s2":  # the full C2 pose set (2.75 million beads) on the same system
+        w = c2()
+        assert np.array_equal(w.coor_rec, s.w.coor_rec)
+        mat = mat_from_qt(w.quat, w.trans).astype(np.float32)
+    _, voxel, _, _ = s.system.bead_terms(mat, s.ligs[0])
+    got = voxel.cpu().numpy().astype(np.int64)
+    ref, edge = _fp64_voxels(mat, s.w.ligands[0].coor, s.grid)
+    flips = got != ref
+    print(f"{which}: {flips.sum()} of {flips.size} bead voxels differ from FP64, nearest boundary of "
+          f"those {edge[flips].max() if flips.any() else 0:.1e} voxel")
+    assert flips.mean() < 1e-3
+    assert np.all(edge[flips] < 1e-4)
