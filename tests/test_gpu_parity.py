@@ -606,3 +606,30 @@ def test_voxel_boundary_flips_vs_fp64(which, request):
           f"those {edge[flips].max() if flips.any() else 0:.1e} voxel")
     assert flips.mean() < 1e-3
     assert np.all(edge[flips] < 1e-4)
+
+
+def test_bead_on_a_receptor_bead_has_infinite_energy(s1):
+    """dsq = 0 inside the plateau: the reference applies no guard (nonbon's 1/dsq is infinite,
+    test-grid-energrad.py:95-111), and neither do the kernels: the pose's energy is +inf, as in
+    the FP64 oracle.  A receptor bead inside the inner grid is hit exactly by a ligand whose first
+    bead sits at its own origin, translated onto it."""
+    from attract_jax_amd.device import Ligand
+    w = s1.w
+    g = s1.grid
+    lo, hi = g.origin + g.gridspacing, g.origin + g.gridspacing * (np.asarray(g.dim) - 2)
+    r = int(np.nonzero(np.all((w.coor_rec > lo) & (w.coor_rec < hi), axis=1))[0][0])
+    coor = np.array([[0.0, 0.0, 0.0], [1.7, 0.4, -0.3]], np.float32)
+    types = np.array([w.ligands[0].types[0], w.ligands[0].types[1]])
+    lig = Ligand.create(coor, types - 1, types - 1, DEV)
+    mat = np.eye(4, dtype=np.float32)[None].repeat(2, 0)
+    mat[0, 3, :3] = w.coor_rec[r].astype(np.float32)          # bead 0 exactly on receptor bead r
+    mat[1, 3, :3] = w.coor_rec[r].astype(np.float32) + 0.37   # beside it: finite
+    E, _ = s1.system.energrad(mat, lig)
+    E = E.cpu().numpy()
+    # the receptor as the kernels (and the FP32 reference) hold it, so that the distance is 0 in FP64 too
+    rec32 = w.coor_rec.astype(np.float32).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ref = O.energrad(mat.astype(np.float64), coor, types - 1, types - 1, rec32, w.rec_types - 1, s1.ff,
+                         s1.grid)[0]
+    assert np.isposinf(E[0]) and np.isposinf(ref[0])
+    assert np.isfinite(E[1]) and abs(E[1] - ref[1]) <= 1e-4 * max(abs(ref[1]), 1.0)
