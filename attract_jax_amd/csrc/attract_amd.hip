@@ -1954,9 +1954,9 @@ __device__ ListBrick list_brick(const GridView& G, float nbdis) {
   return k;
 }
 
-// Stages into s_cand, in receptor order, the beads from *pos on that lie within the brick's
+// Stages into s_cand, in receptor order, the beads from index pos on that lie within the brick's
 // cull radius (x, y, z and the receptor index as int bits), until the next round of
-// blockDim.x beads might not fit or the receptor ends; *pos moves past the beads examined.
+// blockDim.x beads might not fit or the receptor ends; pos moves past the beads examined.
 // Returns the number staged.  Every thread of the block calls it; it ends with a barrier.
 __device__ int stage_candidates(const float4* rec, int nrec, int& pos, const ListBrick& k, float4* s_cand,
                                 int* s_wcount) {
