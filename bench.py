@@ -77,10 +77,31 @@ def scoring_rates(dev, wpb):
     wts = torch.ones(m4.shape[0], device=dev)
     ms4 = best_ms(lambda: param_adjoint(system, ligs[0], m4, system.ff, wts, system.flags), stream)
     log(f"[bench] scoring: C2 energrad {ms2:.3f} ms, C4 adjoint {ms4:.3f} ms")
+    # test5's protocol (test5.py:14-22,450-458: seven stages, 375 adabelief steps with the gravity
+    # restraint in the first four) over the C2 starting poses, one launch per stage
+    from attract_jax_amd.minimize import BatchPlan, PoseState
+    from attract_jax_amd.protocol import SCHEMA_TEST5, run_schedule
+    plan = BatchPlan(system, ligs, [len(w2.quat)], 1, 1e-3, 1e-3)
+    st5 = PoseState.create(w2.quat, w2.trans, dev)
+    init5 = st5.pose.clone()
+
+    def test5_run():
+        st5.pose.copy_(init5)
+        run_schedule(system, plan, st5, SCHEMA_TEST5)
+
+    ms5 = best_ms(test5_run, stream)
+    steps5 = int(sum(e for e, _, _ in SCHEMA_TEST5))
+    log(f"[bench] test5 protocol: {len(w2.quat)} structures x {steps5} steps in {ms5:.1f} ms")
     return {"c2_energrad_poses_per_s": len(w2.quat) / (ms2 * 1e-3), "c2_energrad_ms": ms2, "c2_poses": len(w2.quat),
             "c4_adjoint_poses_per_s": len(w4.quat) / (ms4 * 1e-3), "c4_adjoint_ms": ms4, "c4_poses": len(w4.quat),
             "note": "energies + dE/dM of every pose (C2); sum over poses of dE/d(rc, ac, emin) with its partial-sum "
                     "reduction (C4); best of 3 launches",
+            "test5_protocol": {"structures": len(w2.quat), "steps": steps5, "ms": ms5,
+                               "structures_per_hour": len(w2.quat) / (ms5 * 1e-3) * 3600.0,
+                               "pose_steps_per_s": len(w2.quat) * steps5 / (ms5 * 1e-3),
+                               "workload": "the C2 system's 131,072 starting poses of its 21-bead ligand through "
+                                           "test5's seven-stage adabelief + gravity schedule, one launch per "
+                                           "stage; best of 3"},
             "grid_build": grid_build_times(dev, best_ms, stream)}
 
 
