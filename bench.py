@@ -92,6 +92,14 @@ def scoring_rates(dev, wpb):
     ms5 = best_ms(test5_run, stream)
     steps5 = int(sum(e for e, _, _ in SCHEMA_TEST5))
     log(f"[bench] test5 protocol: {len(w2.quat)} structures x {steps5} steps in {ms5:.1f} ms")
+    # ligand RMSD of every C2 pose against the ligand's own frame (test-rmsd.py:13-17), through the C-ABI
+    from attract_jax_amd.device import _ptr, _stream
+    lib = _lib.load()
+    lig_xyz = torch.as_tensor(np.asarray(w2.ligands[0].coor, np.float32), device=dev).contiguous()
+    rmsd_out = torch.empty(mat.shape[0], dtype=torch.float32, device=dev)
+    mat16 = mat.reshape(-1, 16).contiguous()
+    ms_rmsd = best_ms(lambda: _lib.check(lib.atj_ligand_rmsd(_ptr(mat16), mat16.shape[0], _ptr(lig_xyz), _ptr(lig_xyz),
+                                                             lig_xyz.shape[0], _ptr(rmsd_out), _stream(dev))), stream)
     return {"c2_energrad_poses_per_s": len(w2.quat) / (ms2 * 1e-3), "c2_energrad_ms": ms2, "c2_poses": len(w2.quat),
             "c4_adjoint_poses_per_s": len(w4.quat) / (ms4 * 1e-3), "c4_adjoint_ms": ms4, "c4_poses": len(w4.quat),
             "note": "energies + dE/dM of every pose (C2); sum over poses of dE/d(rc, ac, emin) with its partial-sum "
@@ -102,6 +110,8 @@ def scoring_rates(dev, wpb):
                                "workload": "the C2 system's 131,072 starting poses of its 21-bead ligand through "
                                            "test5's seven-stage adabelief + gravity schedule, one launch per "
                                            "stage; best of 3"},
+            "ligand_rmsd": {"poses": int(mat16.shape[0]), "beads": int(lig_xyz.shape[0]), "ms": ms_rmsd,
+                            "poses_per_s": mat16.shape[0] / (ms_rmsd * 1e-3)},
             "grid_build": grid_build_times(dev, best_ms, stream)}
 
 
